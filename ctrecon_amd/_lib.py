@@ -1,0 +1,97 @@
+"""ctypes binding of libctrecon_amd.so (the C ABI declared in include/ctrecon.h).
+
+The library is loaded from this package folder only.  There is no fallback: if
+the shared object is missing or a call fails, a :class:`CtrError` is raised.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libctrecon_amd.so")
+
+CTR_OK = 0
+CTR_ERR_ARG = 1
+CTR_ERR_HIP = 2
+CTR_ERR_UNSUPPORTED = 3
+CTR_ERR_NOMEM = 4
+CTR_ERR_NUMERIC = 5
+CTR_BP_REFERENCE = 0
+CTR_BP_MATCHED = 1
+
+_NAMES = {0: "CTR_OK", 1: "CTR_ERR_ARG", 2: "CTR_ERR_HIP", 3: "CTR_ERR_UNSUPPORTED",
+          4: "CTR_ERR_NOMEM", 5: "CTR_ERR_NUMERIC"}
+
+# Every symbol include/ctrecon.h declares.
+EXPORTS = [
+    "ctr_dd3_proj", "ctr_dd3_back", "ctr_plan_create", "ctr_plan_destroy", "ctr_plan_fp",
+    "ctr_plan_bp", "ctr_os_sart", "ctr_conjgrad", "ctr_residual", "ctr_device_count",
+    "ctr_malloc", "ctr_free", "ctr_memcpy_h2d", "ctr_memcpy_d2h", "ctr_memset",
+    "ctr_synchronize", "ctr_last_error", "ctr_version",
+]
+
+
+class CtrError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class CtrGeometry(ctypes.Structure):
+    _fields_ = [
+        ("x0", ctypes.c_float), ("y0", ctypes.c_float), ("z0", ctypes.c_float),
+        ("DNU", ctypes.c_int), ("DNV", ctypes.c_int),
+        ("xds", ctypes.c_void_p), ("yds", ctypes.c_void_p), ("zds", ctypes.c_void_p),
+        ("imgXCenter", ctypes.c_float), ("imgYCenter", ctypes.c_float),
+        ("imgZCenter", ctypes.c_float),
+        ("angles", ctypes.c_void_p), ("zshifts", ctypes.c_void_p), ("PN", ctypes.c_int),
+        ("XN", ctypes.c_int), ("YN", ctypes.c_int), ("ZN", ctypes.c_int),
+        ("dx", ctypes.c_float), ("dz", ctypes.c_float),
+        ("mask", ctypes.c_void_p),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libctrecon_amd.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CtrError(CTR_ERR_ARG, f"{LIB_PATH} is missing; run python -m ctrecon_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        f, i, p, d = ctypes.c_float, ctypes.c_int, ctypes.c_void_p, ctypes.c_double
+        geo = [f, f, f, i, i, p, p, p, f, f, f, p, p, i, i, i, i]
+        L.ctr_dd3_proj.argtypes = geo + [p, p, f, f, p, i, i]
+        L.ctr_dd3_back.argtypes = geo + [p, p, f, f, p, i, i, i]
+        L.ctr_plan_create.argtypes = [ctypes.POINTER(CtrGeometry), i, ctypes.POINTER(p)]
+        L.ctr_plan_destroy.argtypes = [p]
+        L.ctr_plan_fp.argtypes = [p, p, p, i, i, i, p]
+        L.ctr_plan_bp.argtypes = [p, p, p, i, i, i, i, p]
+        L.ctr_os_sart.argtypes = [p, p, p, i, i, i, p]
+        L.ctr_conjgrad.argtypes = [p, p, p, i, i, p]
+        L.ctr_residual.argtypes = [p, p, p, ctypes.POINTER(d)]
+        L.ctr_device_count.argtypes = [ctypes.POINTER(i)]
+        L.ctr_malloc.argtypes = [ctypes.POINTER(p), ctypes.c_ulonglong, i]
+        L.ctr_free.argtypes = [p]
+        L.ctr_memcpy_h2d.argtypes = [p, p, ctypes.c_ulonglong]
+        L.ctr_memcpy_d2h.argtypes = [p, p, ctypes.c_ulonglong]
+        L.ctr_memset.argtypes = [p, i, ctypes.c_ulonglong]
+        L.ctr_synchronize.argtypes = [i]
+        L.ctr_last_error.restype = ctypes.c_char_p
+        L.ctr_version.restype = ctypes.c_char_p
+        for name in EXPORTS:
+            fn = getattr(L, name)
+            if name not in ("ctr_last_error", "ctr_version"):
+                fn.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != CTR_OK:
+        raise CtrError(rc, lib().ctr_last_error().decode())
+    return rc

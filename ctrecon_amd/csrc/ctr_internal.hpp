@@ -1,0 +1,97 @@
+// Internal declarations shared by the host plan code and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/ctrecon.h"
+
+namespace ctr {
+
+// Per-view constants (computed on the host in FP64, stored FP32).
+//   (sx, sy, sz)  rotated source position (DD3_GPU.cu:875-878, 1441-1444)
+//   az            (z0 + zshift)/dz + ocz: z index of the source plane offset
+//   xmarch        1 when |cos| <= |sin|: march x-slices (DD3_GPU.cu:914)
+//   bdir          +1 / -1: direction in which the in-plane boundary positions of
+//                 this view increase with the boundary index
+//   wx            1 when |src.x| > |src.y| (FP64 decision): the reference back
+//                 projector then uses the x component as the cell width direction
+//                 (DD3_GPU.cu:1453-1460)
+struct ViewP {
+  float sx, sy, sz, az;
+  int xmarch, bdir, wx, pad1;
+};
+
+// Constants of the reference back projector (DD3_GPU.cu:1428-1437, calDetParas
+// :1189-1217), all in unpadded cell-centre index units.
+struct BpConsts {
+  float S;        // S2D / ddv
+  float ib;       // 1 / dbeta
+  float hz;       // dz / 2
+  float inv_sid;  // 1 / |(x0, y0)|
+  float cU, cV;   // detector centre indices (detCtrIdxU/V)
+  float dx, dz;
+};
+
+// Geometry constants every kernel needs.
+struct Dims {
+  int XN, YN, ZN, DNU, DNV, PN;
+  float ocx, ocy, ocz;  // objCntIdx (DD3_GPU.cu:1029-1031)
+  float dx, dz;
+  float beta0, dbeta_g;  // fan angle of boundary 0 and mean boundary spacing (adjoint guess)
+  float B0, dB;          // row boundary index of row 0 and mean spacing (adjoint guess)
+};
+
+struct Plan {
+  int device = 0;
+  Dims d{};
+  BpConsts bc{};
+  // device tables
+  ViewP* views = nullptr;      // [PN]
+  float2* bnd = nullptr;       // [PN][DNU+1]  (a_b, f_b): t_b(ii) = a_b + ii*f_b
+  float4* cols = nullptr;      // [PN][DNU]    (r0, r1, G, Q)
+  float* Bz = nullptr;         // [DNV+1]      (bz_v - z0)/dz
+  float* h2 = nullptr;         // [DNV]        (zds_v - z0)^2
+  float* invdB = nullptr;      // [DNV]        1/(Bz[v+1]-Bz[v])
+  uint8_t* mask = nullptr;     // [XN*YN]
+  // host copies (for culling and solvers)
+  std::vector<ViewP> h_views;
+  // scratch
+  float* scratch_prj = nullptr;     // [max_views][DNU][DNV]
+  size_t scratch_prj_views = 0;
+  double* partials = nullptr;       // reduction partials
+  size_t partials_n = 0;
+};
+
+// error handling
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define CTR_HIP(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return ::ctr::fail(CTR_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// kernel launchers (kernels.hip)
+int launch_fp(const Plan& P, const float* vol, float* out, int vb, int vs, int vc,
+              hipStream_t st, const float* b, const float* invRow, double* partials);
+int launch_bp(const Plan& P, const float* prj, float* vol, int vb, int vs, int vc, int kind,
+              hipStream_t st, const float* invCol, int accumulate);
+int launch_scale_matched(const Plan& P, const float* in, float* out, int vb, int vs, int vc,
+                         hipStream_t st);
+int launch_fill(float* p, size_t n, float v, hipStream_t st);
+int launch_inv_guard(const float* in, float* out, size_t n, hipStream_t st);
+int launch_axpy(float* y, const float* x, double a, size_t n, hipStream_t st);
+int launch_xpby(float* p, const float* r, double b, size_t n, hipStream_t st);
+int launch_sub(float* out, const float* a, const float* b, size_t n, hipStream_t st);
+int launch_dot(const float* a, const float* b, size_t n, double* partials, int nparts,
+               double* result, hipStream_t st);
+int launch_reduce_pairs(const double* partials, int nparts, double* result, hipStream_t st);
+int fp_partial_blocks(const Plan& P, int vc);
+
+}  // namespace ctr

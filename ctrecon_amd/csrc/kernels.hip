@@ -1,0 +1,489 @@
+// gfx950 kernels of the DD3 projector pair and the solver loops.
+//
+// Work mapping ("z-column waves"): every wave64 owns one column of 64 consecutive
+// z-cells -- in the forward projector the 64 detector rows of one (view, column),
+// in the back projectors 64 voxels along z of one (x, y) column.  All in-plane
+// geometry is then the same for the wave and only the z part is per lane; reads
+// of vol[...][z] and prj[...][v] are contiguous across the lanes.
+//
+// The reference samples 2-D summed-area tables through linear-filtered, clamped
+// textures.  With exact weights that sample equals the integral of the
+// piecewise-constant volume (sinogram) over the footprint rectangle clipped to
+// the array, so the kernels integrate the footprint directly: the same operator,
+// without the FP32 cancellation of whole-plane integral images and without the
+// integral-image build pass.
+#include "ctr_internal.hpp"
+
+namespace ctr {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+
+// Overlap of [lo, hi] with the cell [c - 0.5, c + 0.5].
+__device__ __forceinline__ float cell_w(float lo, float hi, float c) {
+  return fminf(hi, c + 0.5f) - fmaxf(lo, c - 0.5f);
+}
+
+// Block-wide sum of two doubles into partials[2*block + {0,1}].
+__device__ __forceinline__ void block_sum2(double a, double b, double* partials, int block) {
+  __shared__ double red[2][kWaves];
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_xor(a, off, 64);
+    b += __shfl_xor(b, off, 64);
+  }
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = a;
+    red[1][w] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sa = 0, sb = 0;
+    for (int i = 0; i < kWaves; ++i) {
+      sa += red[0][i];
+      sb += red[1][i];
+    }
+    partials[2 * block] = sa;
+    partials[2 * block + 1] = sb;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Forward projector (mode 0 distance-driven), one thread per ray (p, u, v).
+// Reference: DD3_gpu_proj_branchless_sat2d_ker, iterativeRecon/DD3_GPU.cu:844-990.
+// Per slice ii the ray's footprint is [tl, tr] (in-plane) x [zl, zh] (z) in
+// cell-centre index units; its mean over the footprint, summed over slices and
+// scaled by dx/|dir| (K), is the ray sum.
+// FUSED: out = (b - A x) * invRow (os_sart.m:24,30-31; b indexed by global view,
+// invRow by the launch's compact view index, NULL for 1) and per-block sums of
+// (b - A x)^2 and b^2.
+// ---------------------------------------------------------------------------
+template <bool FUSED>
+__global__ __launch_bounds__(kBlock) void fp_kernel(
+    const float* __restrict__ vol, float* __restrict__ out, const ViewP* __restrict__ views,
+    const float2* __restrict__ bnd, const float4* __restrict__ cols,
+    const float* __restrict__ Bz, const float* __restrict__ h2,
+    const float* __restrict__ invdB, const uint8_t* __restrict__ mask, Dims d, int vb, int vs,
+    const float* __restrict__ b, const float* __restrict__ invRow,
+    double* __restrict__ partials) {
+  const int k = blockIdx.y;
+  const int p = vb + k * vs;
+  const int nray = d.DNU * d.DNV;
+  const int ray = blockIdx.x * kBlock + threadIdx.x;
+  double res = 0.0, bsq = 0.0;
+  if (ray < nray) {
+    const int u = ray / d.DNV;
+    const int v = ray - u * d.DNV;
+    const ViewP vp = views[p];
+    const float2 bl = bnd[(size_t)p * (d.DNU + 1) + u];
+    const float2 br = bnd[(size_t)p * (d.DNU + 1) + u + 1];
+    const float4 ct = cols[(size_t)p * d.DNU + u];
+    const float Bv = Bz[v], Bv1 = Bz[v + 1];
+    int NS, NI, sS, cS, mS, mC;
+    if (vp.xmarch) {
+      NS = d.XN; NI = d.YN; sS = d.ZN; cS = d.XN * d.ZN; mS = 1; mC = d.XN;
+    } else {
+      NS = d.YN; NI = d.XN; sS = d.XN * d.ZN; cS = d.ZN; mS = d.XN; mC = 1;
+    }
+    const float NIh = (float)NI - 0.5f, ZNh = (float)d.ZN - 0.5f;
+    float acc = 0.f;
+    for (int ii = 0; ii < NS; ++ii) {
+      const float fi = (float)ii;
+      const float tl = fmaf(fi, bl.y, bl.x);
+      const float tr = fmaf(fi, br.y, br.x);
+      const float r = fmaf(fi, ct.y, ct.x);
+      float ylo = fminf(tl, tr), yhi = fmaxf(tl, tr);
+      const float inv = 1.0f / ((yhi - ylo) * r);
+      ylo = fmaxf(ylo, -0.5f);
+      yhi = fminf(yhi, NIh);
+      if (!(ylo < yhi)) continue;
+      float zl = fmaf(r, Bv, vp.az), zh = fmaf(r, Bv1, vp.az);
+      zl = fmaxf(zl, -0.5f);
+      zh = fminf(zh, ZNh);
+      if (!(zl < zh)) continue;
+      const int k0 = (int)floorf(zl + 0.5f);
+      const int k1 = min((int)floorf(zh + 0.5f), d.ZN - 1);
+      const int j0 = (int)floorf(ylo + 0.5f);
+      const int j1 = min((int)floorf(yhi + 0.5f), NI - 1);
+      float s = 0.f;
+      for (int j = j0; j <= j1; ++j) {
+        const float m = (float)mask[ii * mS + j * mC];
+        const float wy = cell_w(ylo, yhi, (float)j) * m;
+        const float* colp = vol + (size_t)ii * sS + (size_t)j * cS;
+        float g = 0.f;
+        for (int kk = k0; kk <= k1; ++kk) g = fmaf(cell_w(zl, zh, (float)kk), colp[kk], g);
+        s = fmaf(wy, g, s);
+      }
+      acc = fmaf(s, inv, acc);
+    }
+    const float K = ct.w * sqrtf(ct.z + h2[v]) * invdB[v];
+    const float val = acc * K;
+    const size_t o = ((size_t)k * d.DNU + u) * d.DNV + v;
+    if constexpr (FUSED) {
+      const size_t og = ((size_t)p * d.DNU + u) * d.DNV + v;
+      const float bb = b[og];
+      const float diff = bb - val;
+      out[o] = invRow ? diff * invRow[o] : diff;
+      res = (double)diff * (double)diff;
+      bsq = (double)bb * (double)bb;
+    } else {
+      out[o] = val;
+    }
+  }
+  if constexpr (FUSED) block_sum2(res, bsq, partials, blockIdx.y * gridDim.x + blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Reference back projector, one thread per voxel, views looped inside.
+// Reference: DD3_gpu_back_ker<_BRANCHLESS>, iterativeRecon/DD3_GPU.cu:1397-1475.
+// The voxel's shadow in (fan-angle index, row index) space is integrated over
+// the piecewise-constant sinogram and weighted by cosVal.
+// UPDATE: vol += acc * invCol (os_sart.m:32,37-40); otherwise vol = acc.
+// ---------------------------------------------------------------------------
+template <bool UPDATE>
+__global__ __launch_bounds__(kBlock) void bp_ref_kernel(
+    const float* __restrict__ prj, float* __restrict__ vol, const ViewP* __restrict__ views,
+    const uint8_t* __restrict__ mask, Dims d, BpConsts c, int vb, int vs, int vc,
+    const float* __restrict__ invCol) {
+  const int lane = threadIdx.x & 63;
+  const int col = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int z = blockIdx.y * 64 + lane;
+  if (col >= d.XN * d.YN || z >= d.ZN) return;
+  const int y = col / d.XN, x = col - y * d.XN;
+  const size_t o = (size_t)col * d.ZN + z;
+  if (mask[col] != 1) {
+    if constexpr (!UPDATE) vol[o] = 0.f;
+    return;
+  }
+  const float Px = ((float)x - d.ocx) * d.dx;
+  const float Py = ((float)y - d.ocy) * d.dx;
+  const float Pz = ((float)z - d.ocz) * d.dz;
+  const float DNUh = (float)d.DNU - 0.5f, DNVh = (float)d.DNV - 0.5f;
+  const size_t frame = (size_t)d.DNU * d.DNV;
+  float acc = 0.f;
+  for (int k = 0; k < vc; ++k) {
+    const ViewP vp = views[vb + k * vs];
+    const float dX = Px - vp.sx, dY = Py - vp.sy, dZ = Pz - vp.sz;
+    const float l2 = dX * dX + dY * dY;
+    const float rl = rsqrtf(l2);
+    const float tU = (dZ + c.hz) * c.S * rl + c.cV;
+    const float tD = (dZ - c.hz) * c.S * rl + c.cV;
+    const float vlo = fmaxf(fminf(tU, tD), -0.5f);
+    const float vhi = fminf(fmaxf(tU, tD), DNVh);
+    const bool vok = vlo < vhi;
+    if (!__any(vok)) continue;
+    const float alpha = asinf((vp.sy * dX - vp.sx * dY) * c.inv_sid * rl);
+    const float w = vp.wx ? dX : dY;
+    const float dA = w / l2 * c.dx * 0.5f;
+    const float cosVal = c.dx / w * sqrtf(l2 + dZ * dZ);
+    const float tL = (alpha - dA) * c.ib + c.cU;
+    const float tR = (alpha + dA) * c.ib + c.cU;
+    const float ulo = fmaxf(fminf(tL, tR), -0.5f);
+    const float uhi = fminf(fmaxf(tL, tR), DNUh);
+    if (!(ulo < uhi) || !vok) continue;
+    const float sgn = ((tR - tL) * (tU - tD) < 0.f) ? -1.f : 1.f;
+    const int u0 = (int)floorf(ulo + 0.5f);
+    const int u1 = min((int)floorf(uhi + 0.5f), d.DNU - 1);
+    const int v0 = (int)floorf(vlo + 0.5f);
+    const int v1 = min((int)floorf(vhi + 0.5f), d.DNV - 1);
+    const float* pp = prj + (size_t)k * frame;
+    float s = 0.f;
+    for (int u = u0; u <= u1; ++u) {
+      const float* row = pp + (size_t)u * d.DNV;
+      float g = 0.f;
+      for (int v = v0; v <= v1; ++v) g = fmaf(cell_w(vlo, vhi, (float)v), row[v], g);
+      s = fmaf(cell_w(ulo, uhi, (float)u), g, s);
+    }
+    acc = fmaf(sgn * cosVal, s, acc);
+  }
+  if constexpr (UPDATE)
+    vol[o] = fmaf(acc, invCol[o], vol[o]);
+  else
+    vol[o] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Matched adjoint: the exact transpose of fp_kernel, in gather form.
+// Input yp is the sinogram already multiplied by K * invdB (scale_matched_kernel).
+// For each view the voxel's in-plane cell is located among the column footprints
+// on its slice with the same FP32 boundary expressions fp_kernel evaluates, and
+// its z cell among the row footprints; the overlap weights are those of the
+// forward projector.
+// ---------------------------------------------------------------------------
+template <bool UPDATE>
+__global__ __launch_bounds__(kBlock) void bp_matched_kernel(
+    const float* __restrict__ yp, float* __restrict__ vol, const ViewP* __restrict__ views,
+    const float2* __restrict__ bnd, const float4* __restrict__ cols,
+    const float* __restrict__ Bz, const uint8_t* __restrict__ mask, Dims d, BpConsts c,
+    int vb, int vs, int vc, const float* __restrict__ invCol) {
+  const int lane = threadIdx.x & 63;
+  const int col = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int z = blockIdx.y * 64 + lane;
+  if (col >= d.XN * d.YN || z >= d.ZN) return;
+  const int y = col / d.XN, x = col - y * d.XN;
+  const size_t o = (size_t)col * d.ZN + z;
+  const float m = (float)mask[col];
+  if (m == 0.f) {
+    if constexpr (!UPDATE) vol[o] = 0.f;
+    return;
+  }
+  const float Px = ((float)x - d.ocx) * d.dx;
+  const float Py = ((float)y - d.ocy) * d.dx;
+  const float zlo = (float)z - 0.5f, zhi = (float)z + 0.5f;
+  const size_t frame = (size_t)d.DNU * d.DNV;
+  const int DNU = d.DNU, DNV = d.DNV;
+  float acc = 0.f;
+  for (int k = 0; k < vc; ++k) {
+    const int p = vb + k * vs;
+    const ViewP vp = views[p];
+    const int ii = vp.xmarch ? x : y;
+    const int j = vp.xmarch ? y : x;
+    const float fi = (float)ii;
+    const float2* bt = bnd + (size_t)p * (DNU + 1);
+    const float4* ctab = cols + (size_t)p * DNU;
+    const float dir = (float)vp.bdir;
+    const float cl = (vp.bdir > 0) ? (float)j - 0.5f : -((float)j + 0.5f);
+    const float ch = cl + 1.0f;
+    // first guess from the fan angle of the voxel centre
+    const float dX = Px - vp.sx, dY = Py - vp.sy;
+    const float beta = asinf((vp.sy * dX - vp.sx * dY) * c.inv_sid * rsqrtf(dX * dX + dY * dY));
+    int u = (int)fminf(fmaxf(floorf((beta - d.beta0) / d.dbeta_g), 0.f), (float)(DNU - 1));
+    float su = dir * fmaf(fi, bt[u].y, bt[u].x);
+    while (u > 0 && su > cl) {
+      --u;
+      su = dir * fmaf(fi, bt[u].y, bt[u].x);
+    }
+    float su1 = dir * fmaf(fi, bt[u + 1].y, bt[u + 1].x);
+    while (u < DNU - 1 && su1 <= cl) {
+      ++u;
+      su = su1;
+      su1 = dir * fmaf(fi, bt[u + 1].y, bt[u + 1].x);
+    }
+    const float* yk = yp + (size_t)k * frame;
+    for (; u < DNU && su < ch;) {
+      const float wy = fminf(su1, ch) - fmaxf(su, cl);
+      if (wy > 0.f) {
+        const float4 ct = ctab[u];
+        const float r = fmaf(fi, ct.y, ct.x);
+        const float inv = 1.0f / ((su1 - su) * r);
+        // rows whose z footprint [az + Bz[v] r, az + Bz[v+1] r] meets [zlo, zhi]
+        int v = (int)fminf(fmaxf(floorf(((zlo - vp.az) / r - d.B0) / d.dB), 0.f),
+                           (float)(DNV - 1));
+        float zl = fmaf(r, Bz[v], vp.az);
+        while (v > 0 && zl > zlo) {
+          --v;
+          zl = fmaf(r, Bz[v], vp.az);
+        }
+        float zh = fmaf(r, Bz[v + 1], vp.az);
+        while (v < DNV - 1 && zh <= zlo) {
+          ++v;
+          zl = zh;
+          zh = fmaf(r, Bz[v + 1], vp.az);
+        }
+        const float* yu = yk + (size_t)u * DNV;
+        float g = 0.f;
+        for (; v < DNV && zl < zhi;) {
+          const float wz = fminf(zh, zhi) - fmaxf(zl, zlo);
+          if (wz > 0.f) g = fmaf(wz, yu[v], g);
+          ++v;
+          if (v < DNV) {
+            zl = zh;
+            zh = fmaf(r, Bz[v + 1], vp.az);
+          }
+        }
+        acc = fmaf(wy * inv, g, acc);
+      }
+      ++u;
+      if (u < DNU) {
+        su = su1;
+        su1 = dir * fmaf(fi, bt[u + 1].y, bt[u + 1].x);
+      }
+    }
+  }
+  acc *= m;
+  if constexpr (UPDATE)
+    vol[o] = fmaf(acc, invCol[o], vol[o]);
+  else
+    vol[o] = acc;
+}
+
+// yp[k][u][v] = y[k][u][v] * K(p,u,v) * invdB[v]: the per-ray factors of fp_kernel.
+__global__ __launch_bounds__(kBlock) void scale_matched_kernel(
+    const float* __restrict__ in, float* __restrict__ out, const float4* __restrict__ cols,
+    const float* __restrict__ h2, const float* __restrict__ invdB, Dims d, int vb, int vs) {
+  const int k = blockIdx.y;
+  const int p = vb + k * vs;
+  const int ray = blockIdx.x * kBlock + threadIdx.x;
+  if (ray >= d.DNU * d.DNV) return;
+  const int u = ray / d.DNV, v = ray - u * d.DNV;
+  const float4 ct = cols[(size_t)p * d.DNU + u];
+  const float K = ct.w * sqrtf(ct.z + h2[v]) * invdB[v];
+  const size_t o = (size_t)k * d.DNU * d.DNV + ray;
+  out[o] = in[o] * K;
+}
+
+__global__ void fill_kernel(float* __restrict__ p, size_t n, float v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// out = (in != 0) ? 1/in : 1  -- the guarded divisions of os_sart.m:30-31,38-39.
+__global__ void inv_guard_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                 size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const float a = in[i];
+    out[i] = (a != 0.f) ? 1.0f / a : 1.0f;
+  }
+}
+
+__global__ void axpy_kernel(float* __restrict__ y, const float* __restrict__ x, double a,
+                            size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    y[i] = (float)((double)y[i] + a * (double)x[i]);
+}
+
+__global__ void xpby_kernel(float* __restrict__ p, const float* __restrict__ r, double b,
+                            size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = (float)((double)r[i] + b * (double)p[i]);
+}
+
+__global__ void sub_kernel(float* __restrict__ out, const float* __restrict__ a,
+                           const float* __restrict__ b, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    out[i] = a[i] - b[i];
+}
+
+// FP64 dot product: per-block partial sums (fixed order), then one block reduces.
+__global__ __launch_bounds__(kBlock) void dot_partial_kernel(const float* __restrict__ a,
+                                                             const float* __restrict__ b,
+                                                             size_t n,
+                                                             double* __restrict__ partials) {
+  double s = 0.0;
+  for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+    s += (double)a[i] * (double)b[i];
+  block_sum2(s, 0.0, partials, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kBlock) void reduce_pairs_kernel(const double* __restrict__ partials,
+                                                              int nparts,
+                                                              double* __restrict__ result) {
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < nparts; i += kBlock) {
+    a += partials[2 * i];
+    b += partials[2 * i + 1];
+  }
+  block_sum2(a, b, result, 0);
+}
+
+int grid_1d(size_t n) {
+  size_t g = (n + 255) / 256;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+int check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(CTR_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  return CTR_OK;
+}
+
+}  // namespace
+
+int fp_partial_blocks(const Plan& P, int vc) {
+  return ((P.d.DNU * P.d.DNV + kBlock - 1) / kBlock) * vc;
+}
+
+int launch_fp(const Plan& P, const float* vol, float* out, int vb, int vs, int vc,
+              hipStream_t st, const float* b, const float* invRow, double* partials) {
+  if (vc <= 0) return CTR_OK;
+  dim3 grid((P.d.DNU * P.d.DNV + kBlock - 1) / kBlock, vc);
+  if (b) {
+    hipLaunchKernelGGL(fp_kernel<true>, grid, dim3(kBlock), 0, st, vol, out, P.views, P.bnd,
+                       P.cols, P.Bz, P.h2, P.invdB, P.mask, P.d, vb, vs, b, invRow, partials);
+  } else {
+    hipLaunchKernelGGL(fp_kernel<false>, grid, dim3(kBlock), 0, st, vol, out, P.views, P.bnd,
+                       P.cols, P.Bz, P.h2, P.invdB, P.mask, P.d, vb, vs, nullptr, nullptr,
+                       nullptr);
+  }
+  return check_launch("fp_kernel");
+}
+
+int launch_bp(const Plan& P, const float* prj, float* vol, int vb, int vs, int vc, int kind,
+              hipStream_t st, const float* invCol, int accumulate) {
+  dim3 grid((P.d.XN * P.d.YN + kWaves - 1) / kWaves, (P.d.ZN + 63) / 64);
+  if (kind == CTR_BP_MATCHED) {
+    if (accumulate)
+      hipLaunchKernelGGL(bp_matched_kernel<true>, grid, dim3(kBlock), 0, st, prj, vol, P.views,
+                         P.bnd, P.cols, P.Bz, P.mask, P.d, P.bc, vb, vs, vc, invCol);
+    else
+      hipLaunchKernelGGL(bp_matched_kernel<false>, grid, dim3(kBlock), 0, st, prj, vol, P.views,
+                         P.bnd, P.cols, P.Bz, P.mask, P.d, P.bc, vb, vs, vc, nullptr);
+    return check_launch("bp_matched_kernel");
+  }
+  if (accumulate)
+    hipLaunchKernelGGL(bp_ref_kernel<true>, grid, dim3(kBlock), 0, st, prj, vol, P.views, P.mask,
+                       P.d, P.bc, vb, vs, vc, invCol);
+  else
+    hipLaunchKernelGGL(bp_ref_kernel<false>, grid, dim3(kBlock), 0, st, prj, vol, P.views, P.mask,
+                       P.d, P.bc, vb, vs, vc, nullptr);
+  return check_launch("bp_ref_kernel");
+}
+
+int launch_scale_matched(const Plan& P, const float* in, float* out, int vb, int vs, int vc,
+                         hipStream_t st) {
+  if (vc <= 0) return CTR_OK;
+  dim3 grid((P.d.DNU * P.d.DNV + kBlock - 1) / kBlock, vc);
+  hipLaunchKernelGGL(scale_matched_kernel, grid, dim3(kBlock), 0, st, in, out, P.cols, P.h2,
+                     P.invdB, P.d, vb, vs);
+  return check_launch("scale_matched_kernel");
+}
+
+int launch_fill(float* p, size_t n, float v, hipStream_t st) {
+  hipLaunchKernelGGL(fill_kernel, dim3(grid_1d(n)), dim3(256), 0, st, p, n, v);
+  return check_launch("fill_kernel");
+}
+
+int launch_inv_guard(const float* in, float* out, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(inv_guard_kernel, dim3(grid_1d(n)), dim3(256), 0, st, in, out, n);
+  return check_launch("inv_guard_kernel");
+}
+
+int launch_axpy(float* y, const float* x, double a, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(axpy_kernel, dim3(grid_1d(n)), dim3(256), 0, st, y, x, a, n);
+  return check_launch("axpy_kernel");
+}
+
+int launch_xpby(float* p, const float* r, double b, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(xpby_kernel, dim3(grid_1d(n)), dim3(256), 0, st, p, r, b, n);
+  return check_launch("xpby_kernel");
+}
+
+int launch_sub(float* out, const float* a, const float* b, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(sub_kernel, dim3(grid_1d(n)), dim3(256), 0, st, out, a, b, n);
+  return check_launch("sub_kernel");
+}
+
+// result[0] = sum(a*b) in FP64; partials holds 2*nparts doubles.
+int launch_dot(const float* a, const float* b, size_t n, double* partials, int nparts,
+               double* result, hipStream_t st) {
+  hipLaunchKernelGGL(dot_partial_kernel, dim3(nparts), dim3(kBlock), 0, st, a, b, n, partials);
+  int rc = check_launch("dot_partial_kernel");
+  if (rc) return rc;
+  hipLaunchKernelGGL(reduce_pairs_kernel, dim3(1), dim3(kBlock), 0, st, partials, nparts, result);
+  return check_launch("reduce_pairs_kernel");
+}
+
+int launch_reduce_pairs(const double* partials, int nparts, double* result, hipStream_t st) {
+  hipLaunchKernelGGL(reduce_pairs_kernel, dim3(1), dim3(kBlock), 0, st, partials, nparts, result);
+  return check_launch("reduce_pairs_kernel");
+}
+
+}  // namespace ctr

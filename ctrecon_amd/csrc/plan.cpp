@@ -1,0 +1,650 @@
+// Host side of the library: geometry tables (FP64 arithmetic, FP32 storage),
+// the C ABI, and the OS-SART / CG drivers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ctr_internal.hpp"
+
+namespace ctr {
+
+namespace {
+thread_local std::string g_last_error;
+constexpr double kTwoPi = 6.283185307179586;  // utilities.cuh:75
+}  // namespace
+
+const char* last_error_cstr() { return g_last_error.c_str(); }
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+namespace {
+
+// DD3Boundaries (iterativeRecon/utilities.cuh:340-361) in FP64.
+std::vector<double> boundaries(const float* c, int n) {
+  std::vector<double> b(n + 1);
+  if (n + 1 >= 3) {
+    b[0] = 1.5 * c[0] - 0.5 * c[1];
+    for (int i = 1; i < n; ++i) b[i] = 0.5 * c[i - 1] + 0.5 * c[i];
+    b[n] = 1.5 * c[n - 1] - 0.5 * c[n - 2];
+  } else {
+    b[0] = c[0] - 0.5;
+    b[1] = c[0] + 0.5;
+  }
+  return b;
+}
+
+double regularize(double a) {  // utilities.cuh:257-265
+  while (a >= kTwoPi) a -= kTwoPi;
+  while (a < 0) a += kTwoPi;
+  return a;
+}
+
+bool finite_array(const float* a, int n) {
+  for (int i = 0; i < n; ++i)
+    if (!std::isfinite(a[i])) return false;
+  return true;
+}
+
+template <typename T>
+int upload(T** dst, const std::vector<T>& src) {
+  CTR_HIP(hipMalloc((void**)dst, sizeof(T) * std::max<size_t>(src.size(), 1)));
+  CTR_HIP(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+  return CTR_OK;
+}
+
+int validate_geometry(const ctr_geometry* g) {
+  if (!g) return fail(CTR_ERR_ARG, "geometry is NULL");
+  if (g->DNU < 1 || g->DNV < 1 || g->PN < 1 || g->XN < 1 || g->YN < 1 || g->ZN < 1)
+    return fail(CTR_ERR_ARG, "all sizes (DNU, DNV, PN, XN, YN, ZN) must be >= 1");
+  if (!g->xds || !g->yds || !g->zds || !g->angles || !g->zshifts)
+    return fail(CTR_ERR_ARG, "detector or view arrays are NULL");
+  if (!(g->dx > 0.f) || !(g->dz > 0.f)) return fail(CTR_ERR_ARG, "dx and dz must be > 0");
+  if ((long long)g->XN * g->YN * g->ZN > (1LL << 31) - 1)
+    return fail(CTR_ERR_ARG, "volume larger than 2^31-1 voxels");
+  if ((long long)g->DNU * g->DNV > (1LL << 30))
+    return fail(CTR_ERR_ARG, "detector frame too large");
+  if (!finite_array(g->xds, g->DNU) || !finite_array(g->yds, g->DNU) ||
+      !finite_array(g->zds, g->DNV) || !finite_array(g->angles, g->PN) ||
+      !finite_array(g->zshifts, g->PN))
+    return fail(CTR_ERR_NUMERIC, "non-finite geometry value");
+  for (int v = 1; v < g->DNV; ++v)
+    if (!(g->zds[v] > g->zds[v - 1]))
+      return fail(CTR_ERR_UNSUPPORTED, "detector row positions zds must be strictly increasing");
+  return CTR_OK;
+}
+
+}  // namespace
+
+// Builds every table of the plan in FP64 on the host.
+static int build_tables(const ctr_geometry* g, Plan& P) {
+  const int DNU = g->DNU, DNV = g->DNV, PN = g->PN;
+  const double x0 = g->x0, y0 = g->y0, z0 = g->z0, dx = g->dx, dz = g->dz;
+  Dims& d = P.d;
+  d.XN = g->XN; d.YN = g->YN; d.ZN = g->ZN; d.DNU = DNU; d.DNV = DNV; d.PN = PN;
+  // objCntIdx, DD3_GPU.cu:1029-1031
+  const double ocx = (g->XN - 1.0) * 0.5 - g->imgXCenter / dx;
+  const double ocy = (g->YN - 1.0) * 0.5 - g->imgYCenter / dx;
+  const double ocz = (g->ZN - 1.0) * 0.5 - g->imgZCenter / dz;
+  d.ocx = (float)ocx; d.ocy = (float)ocy; d.ocz = (float)ocz;
+  d.dx = (float)dx; d.dz = (float)dz;
+  const std::vector<double> bx = boundaries(g->xds, DNU), by = boundaries(g->yds, DNU),
+                            bz = boundaries(g->zds, DNV);
+
+  // rows
+  std::vector<float> Bz(DNV + 1), h2(DNV), invdB(DNV);
+  for (int v = 0; v <= DNV; ++v) Bz[v] = (float)((bz[v] - z0) / dz);
+  for (int v = 0; v < DNV; ++v) {
+    const double h = g->zds[v] - z0;
+    h2[v] = (float)(h * h);
+    invdB[v] = (float)(dz / (bz[v + 1] - bz[v]));
+  }
+  d.B0 = Bz[0];
+  d.dB = (float)((bz[DNV] - bz[0]) / dz / DNV);
+
+  // fan angles of the column boundaries seen from the source (rotation invariant)
+  const double sid = std::sqrt(x0 * x0 + y0 * y0);
+  auto fan = [&](double px, double py) {
+    const double ex = px - x0, ey = py - y0, l = std::sqrt(ex * ex + ey * ey);
+    return std::asin((y0 * ex - x0 * ey) / (sid * l));
+  };
+  d.beta0 = (float)fan(bx[0], by[0]);
+  d.dbeta_g = (float)((fan(bx[DNU], by[DNU]) - fan(bx[0], by[0])) / DNU);
+  if (!(std::fabs(d.dbeta_g) > 0.f))
+    return fail(CTR_ERR_UNSUPPORTED, "detector columns subtend no fan angle");
+
+  // reference back-projector constants (DD3_GPU.cu:1428-1437, calDetParas :1189-1217)
+  {
+    const double ddv = (bz[DNV] - bz[0]) / DNV;
+    const double cV = (-(bz[0] - z0) / ddv) - 0.5;
+    auto nrm = [](double a, double b, double& oa, double& ob) {
+      const double n = std::sqrt(a * a + b * b);
+      oa = a / n;
+      ob = b / n;
+    };
+    double dirx, diry, lx, ly, rx, ry;
+    nrm(-x0, -y0, dirx, diry);
+    nrm(bx[0] - x0, by[0] - y0, lx, ly);
+    nrm(bx[DNU] - x0, by[DNU] - y0, rx, ry);
+    const double dbeta = std::asin(lx * ry - ly * rx) / DNU;
+    const double minBeta = std::asin(dirx * ly - diry * lx);
+    const double cU = -minBeta / dbeta - 0.5;
+    const double S2D = std::hypot((double)g->xds[0] - x0, (double)g->yds[0] - y0);
+    P.bc.S = (float)(S2D / ddv);
+    P.bc.ib = (float)(1.0 / dbeta);
+    P.bc.hz = (float)(dz * 0.5);
+    P.bc.inv_sid = (float)(1.0 / sid);
+    P.bc.cU = (float)cU;
+    P.bc.cV = (float)cV;
+    P.bc.dx = (float)dx;
+    P.bc.dz = (float)dz;
+  }
+
+  // per-view tables
+  std::vector<ViewP> views(PN);
+  std::vector<float2> bnd((size_t)PN * (DNU + 1));
+  std::vector<float4> cols((size_t)PN * DNU);
+  for (int p = 0; p < PN; ++p) {
+    const double th = regularize((double)g->angles[p]);
+    const double c = std::cos(th), s = std::sin(th);
+    const double sx = x0 * c - y0 * s, sy = x0 * s + y0 * c, sz = z0 + g->zshifts[p];
+    ViewP& vp = views[p];
+    vp.sx = (float)sx; vp.sy = (float)sy; vp.sz = (float)sz;
+    vp.az = (float)(sz / dz + ocz);
+    vp.xmarch = std::fabs(c) <= std::fabs(s) ? 1 : 0;  // DD3_GPU.cu:914
+    vp.wx = std::fabs(sx) > std::fabs(sy) ? 1 : 0;  // DD3_GPU.cu:1453
+    vp.pad1 = 0;
+    const int NS = vp.xmarch ? g->XN : g->YN;
+    const double mid = 0.5 * (NS - 1);
+    for (int b = 0; b <= DNU; ++b) {
+      const double Bx = bx[b] * c - by[b] * s, By = bx[b] * s + by[b] * c;
+      double f, a;
+      if (vp.xmarch) {
+        f = (By - sy) / (Bx - sx);
+        a = ((-ocx * dx - Bx) * f + By) / dx + ocy;
+      } else {
+        f = (Bx - sx) / (By - sy);
+        a = ((-ocy * dx - By) * f + Bx) / dx + ocx;
+      }
+      bnd[(size_t)p * (DNU + 1) + b] = make_float2((float)a, (float)f);
+    }
+    {
+      const float2 b0 = bnd[(size_t)p * (DNU + 1)], bN = bnd[(size_t)p * (DNU + 1) + DNU];
+      vp.bdir = ((bN.x + mid * bN.y) >= (b0.x + mid * b0.y)) ? 1 : -1;
+    }
+    for (int u = 0; u < DNU; ++u) {
+      const double Dx = (double)g->xds[u] * c - (double)g->yds[u] * s;
+      const double Dy = (double)g->xds[u] * s + (double)g->yds[u] * c;
+      const double ex = Dx - sx, ey = Dy - sy;
+      double r0, r1, Q;
+      if (vp.xmarch) {
+        r0 = (-ocx * dx - sx) / ex;
+        r1 = dx / ex;
+        Q = dx / std::fabs(ex);
+      } else {
+        r0 = (-ocy * dx - sy) / ey;
+        r1 = dx / ey;
+        Q = dx / std::fabs(ey);
+      }
+      cols[(size_t)p * DNU + u] = make_float4((float)r0, (float)r1, (float)(ex * ex + ey * ey),
+                                              (float)Q);
+    }
+  }
+  P.h_views = views;
+
+  std::vector<uint8_t> mask((size_t)g->XN * g->YN, 1);
+  if (g->mask) std::memcpy(mask.data(), g->mask, mask.size());
+
+  int rc;
+  if ((rc = upload(&P.views, views))) return rc;
+  if ((rc = upload(&P.bnd, bnd))) return rc;
+  if ((rc = upload(&P.cols, cols))) return rc;
+  if ((rc = upload(&P.Bz, Bz))) return rc;
+  if ((rc = upload(&P.h2, h2))) return rc;
+  if ((rc = upload(&P.invdB, invdB))) return rc;
+  if ((rc = upload(&P.mask, mask))) return rc;
+  return CTR_OK;
+}
+
+static void free_plan(Plan* P) {
+  if (!P) return;
+  (void)hipFree(P->views);
+  (void)hipFree(P->bnd);
+  (void)hipFree(P->cols);
+  (void)hipFree(P->Bz);
+  (void)hipFree(P->h2);
+  (void)hipFree(P->invdB);
+  (void)hipFree(P->mask);
+  (void)hipFree(P->scratch_prj);
+  (void)hipFree(P->partials);
+  delete P;
+}
+
+static int ensure_scratch_prj(Plan& P, size_t views) {
+  if (P.scratch_prj_views >= views) return CTR_OK;
+  (void)hipFree(P.scratch_prj);
+  P.scratch_prj = nullptr;
+  P.scratch_prj_views = 0;
+  const size_t bytes = views * (size_t)P.d.DNU * P.d.DNV * sizeof(float);
+  if (hipMalloc((void**)&P.scratch_prj, bytes) != hipSuccess)
+    return fail(CTR_ERR_NOMEM, "cannot allocate sinogram scratch");
+  P.scratch_prj_views = views;
+  return CTR_OK;
+}
+
+static int ensure_partials(Plan& P, size_t n) {
+  if (P.partials_n >= n) return CTR_OK;
+  (void)hipFree(P.partials);
+  P.partials = nullptr;
+  P.partials_n = 0;
+  if (hipMalloc((void**)&P.partials, n * sizeof(double)) != hipSuccess)
+    return fail(CTR_ERR_NOMEM, "cannot allocate reduction scratch");
+  P.partials_n = n;
+  return CTR_OK;
+}
+
+static int check_views(const Plan& P, int vb, int vs, int vc) {
+  if (vc < 0 || vb < 0 || vs < 1) return fail(CTR_ERR_ARG, "bad view range");
+  if (vc > 0 && (long long)vb + (long long)(vc - 1) * vs >= P.d.PN)
+    return fail(CTR_ERR_ARG, "view range exceeds the plan's views");
+  return CTR_OK;
+}
+
+static int bp_any(Plan& P, const float* prj, float* vol, int vb, int vs, int vc, int kind,
+                  hipStream_t st, const float* invCol, int accumulate) {
+  if (kind == CTR_BP_MATCHED) {
+    int rc = ensure_scratch_prj(P, vc);
+    if (rc) return rc;
+    if ((rc = launch_scale_matched(P, prj, P.scratch_prj, vb, vs, vc, st))) return rc;
+    return launch_bp(P, P.scratch_prj, vol, vb, vs, vc, kind, st, invCol, accumulate);
+  }
+  return launch_bp(P, prj, vol, vb, vs, vc, kind, st, invCol, accumulate);
+}
+
+// RAII device buffer used by the drivers.
+struct DBuf {
+  void* p = nullptr;
+  ~DBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t bytes) {
+    if (hipMalloc(&p, std::max<size_t>(bytes, 1)) != hipSuccess) {
+      p = nullptr;
+      return fail(CTR_ERR_NOMEM, "device allocation failed");
+    }
+    return CTR_OK;
+  }
+  float* f() const { return (float*)p; }
+  double* d() const { return (double*)p; }
+};
+
+}  // namespace ctr
+
+using namespace ctr;
+
+struct ctr_plan {
+  Plan* impl;
+};
+
+namespace {
+
+Plan* get(ctr_plan* p) { return p ? p->impl : nullptr; }
+
+int set_device(int device) {
+  int n = 0;
+  CTR_HIP(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n)
+    return fail(CTR_ERR_ARG, "device " + std::to_string(device) + " not present (" +
+                                 std::to_string(n) + " visible)");
+  CTR_HIP(hipSetDevice(device));
+  return CTR_OK;
+}
+
+ctr_geometry make_geometry(float x0, float y0, float z0, int DNU, int DNV, const float* xds,
+                           const float* yds, const float* zds, float cx, float cy, float cz,
+                           const float* angs, const float* zpos, int PN, int XN, int YN, int ZN,
+                           float dx, float dz, const unsigned char* mask) {
+  ctr_geometry g;
+  g.x0 = x0; g.y0 = y0; g.z0 = z0; g.DNU = DNU; g.DNV = DNV;
+  g.xds = xds; g.yds = yds; g.zds = zds;
+  g.imgXCenter = cx; g.imgYCenter = cy; g.imgZCenter = cz;
+  g.angles = angs; g.zshifts = zpos; g.PN = PN;
+  g.XN = XN; g.YN = YN; g.ZN = ZN; g.dx = dx; g.dz = dz; g.mask = mask;
+  return g;
+}
+
+// prjMode handling of DD3Proj_gpu / DD3Back_gpu (DD3_GPU.cu:1144-1183, 1778-1815):
+// 0 and every value outside 1..3 select the branchless model.
+int check_mode(int mode) {
+  if (mode == 1 || mode == 2 || mode == 3)
+    return fail(CTR_ERR_UNSUPPORTED, "prjMode " + std::to_string(mode) +
+                                         " (pseudo-DD / double / z-line) is not implemented");
+  return CTR_OK;
+}
+
+struct PlanGuard {
+  ctr_plan* p = nullptr;
+  ~PlanGuard() {
+    if (p) ctr_plan_destroy(p);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+const char* ctr_last_error(void) { return last_error_cstr(); }
+
+const char* ctr_version(void) { return "ctrecon_amd 0.1 (gfx950)"; }
+
+int ctr_plan_create(const ctr_geometry* geom, int device, ctr_plan** out) {
+  if (!out) return fail(CTR_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  int rc = validate_geometry(geom);
+  if (rc) return rc;
+  if ((rc = set_device(device))) return rc;
+  Plan* P = new Plan();
+  P->device = device;
+  if ((rc = build_tables(geom, *P))) {
+    free_plan(P);
+    return rc;
+  }
+  *out = new ctr_plan{P};
+  set_error("");
+  return CTR_OK;
+}
+
+int ctr_plan_destroy(ctr_plan* plan) {
+  if (!plan) return CTR_OK;
+  if (plan->impl) {
+    (void)hipSetDevice(plan->impl->device);
+    free_plan(plan->impl);
+  }
+  delete plan;
+  return CTR_OK;
+}
+
+int ctr_plan_fp(ctr_plan* plan, const float* d_vol, float* d_prj, int view_begin,
+                int view_stride, int view_count, void* stream) {
+  Plan* P = get(plan);
+  if (!P || !d_vol || !d_prj) return fail(CTR_ERR_ARG, "NULL plan or buffer");
+  int rc = check_views(*P, view_begin, view_stride, view_count);
+  if (rc) return rc;
+  CTR_HIP(hipSetDevice(P->device));
+  return launch_fp(*P, d_vol, d_prj, view_begin, view_stride, view_count, (hipStream_t)stream,
+                   nullptr, nullptr, nullptr);
+}
+
+int ctr_plan_bp(ctr_plan* plan, const float* d_prj, float* d_vol, int view_begin,
+                int view_stride, int view_count, int bp_kind, void* stream) {
+  Plan* P = get(plan);
+  if (!P || !d_vol || !d_prj) return fail(CTR_ERR_ARG, "NULL plan or buffer");
+  if (bp_kind != CTR_BP_REFERENCE && bp_kind != CTR_BP_MATCHED)
+    return fail(CTR_ERR_ARG, "bp_kind must be CTR_BP_REFERENCE or CTR_BP_MATCHED");
+  int rc = check_views(*P, view_begin, view_stride, view_count);
+  if (rc) return rc;
+  CTR_HIP(hipSetDevice(P->device));
+  return bp_any(*P, d_prj, d_vol, view_begin, view_stride, view_count, bp_kind,
+                (hipStream_t)stream, nullptr, 0);
+}
+
+int ctr_residual(ctr_plan* plan, const float* d_x, const float* d_prj, double* out) {
+  Plan* P = get(plan);
+  if (!P || !d_x || !d_prj || !out) return fail(CTR_ERR_ARG, "NULL argument");
+  CTR_HIP(hipSetDevice(P->device));
+  const int PN = P->d.PN;
+  // chunk the views so the scratch stays modest
+  const int chunk = std::min(PN, 256);
+  int rc = ensure_scratch_prj(*P, chunk);
+  if (rc) return rc;
+  const int nb = fp_partial_blocks(*P, chunk);
+  if ((rc = ensure_partials(*P, 2 * (size_t)nb + 2))) return rc;
+  DBuf sums;
+  if ((rc = sums.alloc(2 * sizeof(double)))) return rc;
+  double rr = 0, bb = 0;
+  for (int v0 = 0; v0 < PN; v0 += chunk) {
+    const int vc = std::min(chunk, PN - v0);
+    if ((rc = launch_fp(*P, d_x, P->scratch_prj, v0, 1, vc, nullptr, d_prj, nullptr,
+                        P->partials)))
+      return rc;
+    if ((rc = launch_reduce_pairs(P->partials, fp_partial_blocks(*P, vc), sums.d(), nullptr)))
+      return rc;
+    double h[2];
+    CTR_HIP(hipMemcpy(h, sums.p, sizeof(h), hipMemcpyDeviceToHost));
+    rr += h[0];
+    bb += h[1];
+  }
+  *out = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+  return CTR_OK;
+}
+
+int ctr_os_sart(ctr_plan* plan, float* d_x, const float* d_prj, int osNum, int iterNum,
+                int bp_kind, double* resid) {
+  Plan* P = get(plan);
+  if (!P || !d_x || !d_prj) return fail(CTR_ERR_ARG, "NULL argument");
+  if (osNum < 1 || osNum > P->d.PN || iterNum < 0)
+    return fail(CTR_ERR_ARG, "osNum must be in [1, PN] and iterNum >= 0");
+  if (bp_kind != CTR_BP_REFERENCE && bp_kind != CTR_BP_MATCHED)
+    return fail(CTR_ERR_ARG, "bad bp_kind");
+  CTR_HIP(hipSetDevice(P->device));
+  const Dims& d = P->d;
+  const size_t NV = (size_t)d.XN * d.YN * d.ZN, frame = (size_t)d.DNU * d.DNV;
+  const int maxV = (d.PN + osNum - 1) / osNum;
+  int rc;
+  DBuf rowInv, colInv, tp, ones, sums;
+  if ((rc = rowInv.alloc(frame * d.PN * sizeof(float)))) return rc;  // subset-compact order
+  if ((rc = colInv.alloc(NV * osNum * sizeof(float)))) return rc;
+  if ((rc = tp.alloc(frame * maxV * sizeof(float)))) return rc;
+  if ((rc = ones.alloc(std::max(NV, frame * maxV) * sizeof(float)))) return rc;
+  if ((rc = sums.alloc(2 * sizeof(double) * osNum))) return rc;
+  if ((rc = ensure_partials(*P, 2 * (size_t)fp_partial_blocks(*P, maxV) + 2))) return rc;
+  hipStream_t st = nullptr;
+  // RowSum{s} = fp(ones), ColSum{s} = bp(ones) (os_sart.m:14-18)
+  if ((rc = launch_fill(ones.f(), std::max(NV, frame * maxV), 1.0f, st))) return rc;
+  std::vector<size_t> rowOff(osNum);
+  size_t off = 0;
+  for (int s = 0; s < osNum; ++s) {
+    const int vc = (d.PN - s + osNum - 1) / osNum;
+    rowOff[s] = off;
+    float* rI = rowInv.f() + off * frame;
+    if ((rc = launch_fp(*P, ones.f(), rI, s, osNum, vc, st, nullptr, nullptr, nullptr))) return rc;
+    if ((rc = launch_inv_guard(rI, rI, frame * vc, st))) return rc;
+    float* cI = colInv.f() + NV * s;
+    if ((rc = bp_any(*P, ones.f(), cI, s, osNum, vc, bp_kind, st, nullptr, 0))) return rc;
+    if ((rc = launch_inv_guard(cI, cI, NV, st))) return rc;
+    off += vc;
+  }
+  for (int it = 0; it < iterNum; ++it) {
+    for (int s = 0; s < osNum; ++s) {
+      const int vc = (d.PN - s + osNum - 1) / osNum;
+      // tp = (prj - A_s x) ./ RowSum (os_sart.m:24,30-31)
+      if ((rc = launch_fp(*P, d_x, tp.f(), s, osNum, vc, st, d_prj, rowInv.f() + rowOff[s] * frame,
+                          P->partials)))
+        return rc;
+      if ((rc = launch_reduce_pairs(P->partials, fp_partial_blocks(*P, vc), sums.d() + 2 * s, st)))
+        return rc;
+      // x += B_s(tp) ./ ColSum (os_sart.m:32,37-40)
+      if ((rc = bp_any(*P, tp.f(), d_x, s, osNum, vc, bp_kind, st, colInv.f() + NV * s, 1)))
+        return rc;
+    }
+    std::vector<double> h(2 * osNum);
+    CTR_HIP(hipMemcpy(h.data(), sums.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    double rr = 0, bb = 0;
+    for (int s = 0; s < osNum; ++s) {
+      rr += h[2 * s];
+      bb += h[2 * s + 1];
+    }
+    if (resid) resid[it] = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+  }
+  CTR_HIP(hipDeviceSynchronize());
+  return CTR_OK;
+}
+
+int ctr_conjgrad(ctr_plan* plan, float* d_x, const float* d_prj, int iterNum, int bp_kind,
+                 double* resid) {
+  Plan* P = get(plan);
+  if (!P || !d_x || !d_prj) return fail(CTR_ERR_ARG, "NULL argument");
+  if (iterNum < 0) return fail(CTR_ERR_ARG, "iterNum must be >= 0");
+  if (bp_kind != CTR_BP_REFERENCE && bp_kind != CTR_BP_MATCHED)
+    return fail(CTR_ERR_ARG, "bad bp_kind");
+  CTR_HIP(hipSetDevice(P->device));
+  const Dims& d = P->d;
+  const size_t NV = (size_t)d.XN * d.YN * d.ZN, NS = (size_t)d.DNU * d.DNV * d.PN;
+  int rc;
+  DBuf r, pv, t, sino, dots;
+  if ((rc = r.alloc(NV * sizeof(float)))) return rc;
+  if ((rc = pv.alloc(NV * sizeof(float)))) return rc;
+  if ((rc = t.alloc(NV * sizeof(float)))) return rc;
+  if ((rc = sino.alloc(NS * sizeof(float)))) return rc;
+  if ((rc = dots.alloc(2 * sizeof(double)))) return rc;
+  const int nparts = 1024;
+  if ((rc = ensure_partials(*P, 2 * (size_t)nparts + 2))) return rc;
+  hipStream_t st = nullptr;
+  auto A = [&](const float* in, float* out) -> int {  // Back(Proj(.)) (conjgrad.m:4-6)
+    int e = launch_fp(*P, in, sino.f(), 0, 1, d.PN, st, nullptr, nullptr, nullptr);
+    if (e) return e;
+    return bp_any(*P, sino.f(), out, 0, 1, d.PN, bp_kind, st, nullptr, 0);
+  };
+  auto dot = [&](const float* a, const float* b, double* outv) -> int {
+    int e = launch_dot(a, b, NV, P->partials, nparts, dots.d(), st);
+    if (e) return e;
+    double h[2];
+    if (hipMemcpy(h, dots.p, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess)
+      return fail(CTR_ERR_HIP, "dot readback failed");
+    *outv = h[0];
+    return CTR_OK;
+  };
+  // b = Back(prj); r0 = b - A(x0); p0 = r0 (conjgrad.m:7-9)
+  if ((rc = bp_any(*P, d_prj, r.f(), 0, 1, d.PN, bp_kind, st, nullptr, 0))) return rc;
+  if ((rc = A(d_x, t.f()))) return rc;
+  if ((rc = launch_sub(r.f(), r.f(), t.f(), NV, st))) return rc;
+  CTR_HIP(hipMemcpyAsync(pv.p, r.p, NV * sizeof(float), hipMemcpyDeviceToDevice, st));
+  double rr;
+  if ((rc = dot(r.f(), r.f(), &rr))) return rc;
+  for (int it = 0; it < iterNum; ++it) {
+    if ((rc = A(pv.f(), t.f()))) return rc;  // conjgrad.m:14
+    double pAp;
+    if ((rc = dot(pv.f(), t.f(), &pAp))) return rc;
+    const double alpha = rr / pAp;  // conjgrad.m:18
+    if ((rc = launch_axpy(d_x, pv.f(), alpha, NV, st))) return rc;    // :19
+    if ((rc = launch_axpy(r.f(), t.f(), -alpha, NV, st))) return rc;  // :20
+    double r1;
+    if ((rc = dot(r.f(), r.f(), &r1))) return rc;
+    const double beta = r1 / rr;  // :24
+    if ((rc = launch_xpby(pv.f(), r.f(), beta, NV, st))) return rc;  // :25
+    rr = r1;
+    if (resid) resid[it] = std::sqrt(r1);
+  }
+  CTR_HIP(hipDeviceSynchronize());
+  return CTR_OK;
+}
+
+int ctr_dd3_proj(float x0, float y0, float z0, int DNU, int DNV, const float* xds,
+                 const float* yds, const float* zds, float imgXCenter, float imgYCenter,
+                 float imgZCenter, const float* hangs, const float* hzPos, int PN, int XN,
+                 int YN, int ZN, const float* hvol, float* hprj, float dx, float dz,
+                 const unsigned char* mask, int gpunum, int prjMode) {
+  int rc = check_mode(prjMode);
+  if (rc) return rc;
+  if (!hvol || !hprj) return fail(CTR_ERR_ARG, "NULL volume or projection buffer");
+  ctr_geometry g = make_geometry(x0, y0, z0, DNU, DNV, xds, yds, zds, imgXCenter, imgYCenter,
+                                 imgZCenter, hangs, hzPos, PN, XN, YN, ZN, dx, dz, mask);
+  PlanGuard pg;
+  if ((rc = ctr_plan_create(&g, gpunum, &pg.p))) return rc;
+  const size_t NV = (size_t)XN * YN * ZN, NS = (size_t)DNU * DNV * PN;
+  DBuf dv, dp;
+  if ((rc = dv.alloc(NV * sizeof(float)))) return rc;
+  if ((rc = dp.alloc(NS * sizeof(float)))) return rc;
+  CTR_HIP(hipMemcpy(dv.p, hvol, NV * sizeof(float), hipMemcpyHostToDevice));
+  if ((rc = ctr_plan_fp(pg.p, dv.f(), dp.f(), 0, 1, PN, nullptr))) return rc;
+  CTR_HIP(hipMemcpy(hprj, dp.p, NS * sizeof(float), hipMemcpyDeviceToHost));
+  return CTR_OK;
+}
+
+int ctr_dd3_back(float x0, float y0, float z0, int DNU, int DNV, const float* xds,
+                 const float* yds, const float* zds, float imgXCenter, float imgYCenter,
+                 float imgZCenter, const float* hangs, const float* hzPos, int PN, int XN,
+                 int YN, int ZN, float* hvol, const float* hprj, float dx, float dz,
+                 const unsigned char* mask, int gpunum, int squared, int prjMode) {
+  (void)squared;  // ignored, as in the reference (DD3_GPU.cu:1393)
+  const int kind = (prjMode & 16) ? CTR_BP_MATCHED : CTR_BP_REFERENCE;
+  int rc = check_mode(prjMode & ~16);
+  if (rc) return rc;
+  if (!hvol || !hprj) return fail(CTR_ERR_ARG, "NULL volume or projection buffer");
+  ctr_geometry g = make_geometry(x0, y0, z0, DNU, DNV, xds, yds, zds, imgXCenter, imgYCenter,
+                                 imgZCenter, hangs, hzPos, PN, XN, YN, ZN, dx, dz, mask);
+  PlanGuard pg;
+  if ((rc = ctr_plan_create(&g, gpunum, &pg.p))) return rc;
+  const size_t NV = (size_t)XN * YN * ZN, NS = (size_t)DNU * DNV * PN;
+  DBuf dv, dp;
+  if ((rc = dv.alloc(NV * sizeof(float)))) return rc;
+  if ((rc = dp.alloc(NS * sizeof(float)))) return rc;
+  CTR_HIP(hipMemcpy(dp.p, hprj, NS * sizeof(float), hipMemcpyHostToDevice));
+  if ((rc = ctr_plan_bp(pg.p, dp.f(), dv.f(), 0, 1, PN, kind, nullptr))) return rc;
+  CTR_HIP(hipMemcpy(hvol, dv.p, NV * sizeof(float), hipMemcpyDeviceToHost));
+  return CTR_OK;
+}
+
+int ctr_device_count(int* count) {
+  if (!count) return fail(CTR_ERR_ARG, "count is NULL");
+  *count = 0;
+  hipError_t e = hipGetDeviceCount(count);
+  if (e != hipSuccess) {
+    *count = 0;
+    return fail(CTR_ERR_HIP, std::string("hipGetDeviceCount: ") + hipGetErrorString(e));
+  }
+  return CTR_OK;
+}
+
+int ctr_malloc(void** d_ptr, unsigned long long bytes, int device) {
+  if (!d_ptr) return fail(CTR_ERR_ARG, "d_ptr is NULL");
+  int rc = set_device(device);
+  if (rc) return rc;
+  if (hipMalloc(d_ptr, bytes ? bytes : 1) != hipSuccess) {
+    *d_ptr = nullptr;
+    return fail(CTR_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+  }
+  return CTR_OK;
+}
+
+int ctr_free(void* d_ptr) {
+  if (d_ptr) CTR_HIP(hipFree(d_ptr));
+  return CTR_OK;
+}
+
+int ctr_memcpy_h2d(void* d_dst, const void* h_src, unsigned long long bytes) {
+  if (!d_dst || !h_src) return fail(CTR_ERR_ARG, "NULL pointer");
+  CTR_HIP(hipMemcpy(d_dst, h_src, bytes, hipMemcpyHostToDevice));
+  return CTR_OK;
+}
+
+int ctr_memcpy_d2h(void* h_dst, const void* d_src, unsigned long long bytes) {
+  if (!h_dst || !d_src) return fail(CTR_ERR_ARG, "NULL pointer");
+  CTR_HIP(hipMemcpy(h_dst, d_src, bytes, hipMemcpyDeviceToHost));
+  return CTR_OK;
+}
+
+int ctr_memset(void* d_dst, int value, unsigned long long bytes) {
+  if (!d_dst) return fail(CTR_ERR_ARG, "NULL pointer");
+  CTR_HIP(hipMemset(d_dst, value, bytes));
+  return CTR_OK;
+}
+
+int ctr_synchronize(int device) {
+  int rc = set_device(device);
+  if (rc) return rc;
+  CTR_HIP(hipDeviceSynchronize());
+  return CTR_OK;
+}
+
+}  // extern "C"

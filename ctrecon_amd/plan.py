@@ -1,0 +1,176 @@
+"""Device-resident plan and device buffers over the C ABI.
+
+:class:`DeviceArray` owns device memory allocated by the library itself (no
+other GPU runtime is needed in Python).  :class:`Plan` holds one scan's geometry
+tables on the GPU and exposes the forward / back projectors and the OS-SART / CG
+drivers on device buffers.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import CTR_BP_MATCHED, CTR_BP_REFERENCE, CtrGeometry, check, lib
+from .geometry import Geometry
+
+__all__ = ["DeviceArray", "Plan", "device_count", "bp_kind_code"]
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    rc = lib().ctr_device_count(ctypes.byref(n))
+    return n.value if rc == 0 else 0
+
+
+def bp_kind_code(kind) -> int:
+    if kind in ("ref", "reference", CTR_BP_REFERENCE):
+        return CTR_BP_REFERENCE
+    if kind in ("matched", "adjoint", CTR_BP_MATCHED):
+        return CTR_BP_MATCHED
+    raise ValueError(f"unknown back projector {kind!r}")
+
+
+class DeviceArray:
+    """A float32 (or other dtype) buffer in device memory."""
+
+    def __init__(self, shape, dtype=np.float32, device=0):
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.device = device
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        check(lib().ctr_malloc(ctypes.byref(p), self.nbytes, device))
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a, device=0):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype, device)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        check(lib().ctr_memcpy_h2d(self.ptr, a.ctypes.data, self.nbytes))
+
+    def get(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        check(lib().ctr_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def zero(self):
+        check(lib().ctr_memset(self.ptr, 0, self.nbytes))
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().ctr_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Plan:
+    """Geometry tables of one scan on one GPU (``ctr_plan_create``)."""
+
+    def __init__(self, g: Geometry, device: int = 0):
+        self.g = g
+        self.device = device
+        self._keep = [np.ascontiguousarray(a, dtype=np.float32)
+                      for a in (g.xds, g.yds, g.zds, g.angles, g.zshifts)]
+        self._mask = np.ascontiguousarray(g.mask, dtype=np.uint8) if g.mask is not None else None
+        cg = CtrGeometry(
+            x0=g.x0, y0=g.y0, z0=g.z0, DNU=g.DNU, DNV=g.DNV,
+            xds=self._keep[0].ctypes.data, yds=self._keep[1].ctypes.data,
+            zds=self._keep[2].ctypes.data, imgXCenter=g.imgXCenter, imgYCenter=g.imgYCenter,
+            imgZCenter=g.imgZCenter, angles=self._keep[3].ctypes.data,
+            zshifts=self._keep[4].ctypes.data, PN=g.PN, XN=g.XN, YN=g.YN, ZN=g.ZN, dx=g.dx,
+            dz=g.dz, mask=self._mask.ctypes.data if self._mask is not None else None)
+        h = ctypes.c_void_p()
+        check(lib().ctr_plan_create(ctypes.byref(cg), device, ctypes.byref(h)))
+        self.handle = h.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().ctr_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- device-buffer API ----
+    def fp(self, d_vol: DeviceArray, d_prj: DeviceArray, view_begin=0, view_stride=1,
+           view_count=None, stream=None):
+        if view_count is None:
+            view_count = len(range(view_begin, self.g.PN, view_stride))
+        check(lib().ctr_plan_fp(self.handle, d_vol.ptr, d_prj.ptr, view_begin, view_stride,
+                                view_count, stream))
+
+    def bp(self, d_prj: DeviceArray, d_vol: DeviceArray, view_begin=0, view_stride=1,
+           view_count=None, kind="ref", stream=None):
+        if view_count is None:
+            view_count = len(range(view_begin, self.g.PN, view_stride))
+        check(lib().ctr_plan_bp(self.handle, d_prj.ptr, d_vol.ptr, view_begin, view_stride,
+                                view_count, bp_kind_code(kind), stream))
+
+    def os_sart(self, d_x: DeviceArray, d_prj: DeviceArray, osNum: int, iterNum: int,
+                kind="ref"):
+        res = np.zeros(max(iterNum, 1), dtype=np.float64)
+        check(lib().ctr_os_sart(self.handle, d_x.ptr, d_prj.ptr, osNum, iterNum,
+                                bp_kind_code(kind), res.ctypes.data))
+        return res[:iterNum]
+
+    def conjgrad(self, d_x: DeviceArray, d_prj: DeviceArray, iterNum: int, kind="ref"):
+        res = np.zeros(max(iterNum, 1), dtype=np.float64)
+        check(lib().ctr_conjgrad(self.handle, d_x.ptr, d_prj.ptr, iterNum, bp_kind_code(kind),
+                                 res.ctypes.data))
+        return res[:iterNum]
+
+    def residual(self, d_x: DeviceArray, d_prj: DeviceArray) -> float:
+        out = ctypes.c_double()
+        check(lib().ctr_residual(self.handle, d_x.ptr, d_prj.ptr, ctypes.byref(out)))
+        return out.value
+
+    def synchronize(self):
+        check(lib().ctr_synchronize(self.device))
+
+    # ---- host-array conveniences ----
+    def forward(self, vol, views=None):
+        """Host vol[y, x, z] -> host prj[k, u, v] for views (all, or a strided range)."""
+        vb, vs, vc = _view_range(self.g.PN, views)
+        dv = DeviceArray.from_host(np.asarray(vol, np.float32), self.device)
+        dp = DeviceArray((vc, self.g.DNU, self.g.DNV), np.float32, self.device)
+        self.fp(dv, dp, vb, vs, vc)
+        return dp.get()
+
+    def back(self, prj, views=None, kind="ref"):
+        vb, vs, vc = _view_range(self.g.PN, views)
+        dp = DeviceArray.from_host(np.asarray(prj, np.float32), self.device)
+        dv = DeviceArray(self.g.vol_shape, np.float32, self.device)
+        self.bp(dp, dv, vb, vs, vc, kind)
+        return dv.get()
+
+
+def _view_range(PN, views):
+    """(begin, stride, count) of a strided view range (None = all views)."""
+    if views is None:
+        return 0, 1, PN
+    if isinstance(views, range):
+        return views.start, views.step, len(views)
+    v = np.asarray(views, dtype=np.int64)
+    if v.size == 1:
+        return int(v[0]), 1, 1
+    st = int(v[1] - v[0])
+    if st < 1 or not np.all(np.diff(v) == st):
+        raise ValueError("views must be an increasing arithmetic progression")
+    return int(v[0]), st, int(v.size)

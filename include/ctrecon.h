@@ -1,0 +1,140 @@
+/*
+ * ctrecon.h -- C ABI of the MI355X DD3 projector library (libctrecon_amd.so).
+ *
+ * Drop-in boundary for the reference's DD3 GPU projector pair and the OS-SART /
+ * CG loops that drive it (LittleRabbitRun/CTRecon).  Plain pointers and sizes
+ * only; every entry point returns a status code (CTR_OK == 0) and never exits
+ * the process; ctr_last_error() gives a thread-local message.
+ *
+ * Data layouts (as in the reference GPU code):
+ *   volume    vol[(y*XN + x)*ZN + z]         (iterativeRecon/DD3_GPU.cu:44,1473)
+ *   sinogram  prj[(p*DNU + u)*DNV + v]       (iterativeRecon/DD3_GPU.cu:951)
+ *   mask      mask[y*XN + x], unsigned bytes (iterativeRecon/DD3_GPU.cu:1004-1011,1417)
+ * Geometry arguments are those of the GPU path after dd3.m's axis swap
+ * (angles already +pi/2, iterativeRecon/dd3.m:247-250,305-307).
+ */
+#ifndef CTRECON_H
+#define CTRECON_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define CTR_OK 0
+#define CTR_ERR_ARG 1         /* invalid argument (null pointer, bad size, bad mode) */
+#define CTR_ERR_HIP 2         /* HIP runtime error */
+#define CTR_ERR_UNSUPPORTED 3 /* projector mode or geometry this build does not implement */
+#define CTR_ERR_NOMEM 4       /* device allocation failed */
+#define CTR_ERR_NUMERIC 5     /* non-finite input where a finite one is required */
+
+/* ---- projector modes (prjMode argument of the reference, iterativeRecon/dd3.m:233-246) ---- */
+#define CTR_MODE_BRANCHLESS 0   /* distance-driven, branchless (the reference's mode 0 / default) */
+/* Back projector selector (build addition, no reference mode number): the exact
+ * transpose of the mode-0 forward projector, used for adjoint tests and CG. */
+#define CTR_BP_REFERENCE 0
+#define CTR_BP_MATCHED 1
+
+/*
+ * Forward projection, host buffers in and out.
+ * Replaces DD3Proj_gpu (iterativeRecon/DD3_GPU.cu:1132-1142), same parameter list.
+ * Unlike the reference, hvol is not modified (the mask is applied on the device)
+ * and the device is not reset.  prjMode 0 (and any value the reference maps to
+ * its default branch) selects the branchless distance-driven model; modes 1-3
+ * return CTR_ERR_UNSUPPORTED.
+ */
+int ctr_dd3_proj(float x0, float y0, float z0, int DNU, int DNV, const float* xds,
+                 const float* yds, const float* zds, float imgXCenter, float imgYCenter,
+                 float imgZCenter, const float* hangs, const float* hzPos, int PN, int XN,
+                 int YN, int ZN, const float* hvol, float* hprj, float dx, float dz,
+                 const unsigned char* mask, int gpunum, int prjMode);
+
+/*
+ * Back projection, host buffers in and out.
+ * Replaces DD3Back_gpu (iterativeRecon/DD3_GPU.cu:1766-1775), same parameter list
+ * (squared is accepted and ignored, as in the reference, :1393).  prjMode 0 is the
+ * reference-arithmetic branchless back projector; bit 4 (prjMode | 16) selects the
+ * matched adjoint instead.  Voxels whose mask byte is not 1 are 0 (:1417).
+ */
+int ctr_dd3_back(float x0, float y0, float z0, int DNU, int DNV, const float* xds,
+                 const float* yds, const float* zds, float imgXCenter, float imgYCenter,
+                 float imgZCenter, const float* hangs, const float* hzPos, int PN, int XN,
+                 int YN, int ZN, float* hvol, const float* hprj, float dx, float dz,
+                 const unsigned char* mask, int gpunum, int squared, int prjMode);
+
+/* ---- persistent, device-resident plan ---- */
+
+/* The geometry of one scan: the non-data arguments of DD3Proj_gpu
+ * (iterativeRecon/DD3_GPU.cu:1133-1142). */
+typedef struct ctr_geometry {
+  float x0, y0, z0;
+  int DNU, DNV;
+  const float* xds; /* [DNU] */
+  const float* yds; /* [DNU] */
+  const float* zds; /* [DNV], strictly increasing */
+  float imgXCenter, imgYCenter, imgZCenter;
+  const float* angles;  /* [PN] */
+  const float* zshifts; /* [PN] */
+  int PN;
+  int XN, YN, ZN;
+  float dx, dz;
+  const unsigned char* mask; /* [XN*YN] or NULL for all ones */
+} ctr_geometry;
+
+typedef struct ctr_plan ctr_plan;
+
+/* Build per-view tables on `device` (host FP64 arithmetic, stored FP32).
+ * Replaces the per-call set-up of DD3_gpu_proj_branchless_sat2d / DD3_gpu_back
+ * (iterativeRecon/DD3_GPU.cu:1012-1079, 1664-1686), done once per scan. */
+int ctr_plan_create(const ctr_geometry* geom, int device, ctr_plan** out);
+int ctr_plan_destroy(ctr_plan* plan);
+
+/* Forward-project device volume d_vol into device sinogram d_prj for the views
+ * view_begin + k*view_stride, k < view_count; d_prj is [view_count][DNU][DNV].
+ * The kernel of DD3_gpu_proj_branchless_sat2d_ker (iterativeRecon/DD3_GPU.cu:844-990).
+ * stream is a hipStream_t (NULL: the null stream). Asynchronous. */
+int ctr_plan_fp(ctr_plan* plan, const float* d_vol, float* d_prj, int view_begin,
+                int view_stride, int view_count, void* stream);
+
+/* Back-project device sinogram d_prj ([view_count][DNU][DNV], views as in
+ * ctr_plan_fp) into device volume d_vol (overwritten).  bp_kind: CTR_BP_REFERENCE
+ * (DD3_gpu_back_ker<_BRANCHLESS>, iterativeRecon/DD3_GPU.cu:1397-1475) or
+ * CTR_BP_MATCHED (exact transpose of ctr_plan_fp).  Asynchronous. */
+int ctr_plan_bp(ctr_plan* plan, const float* d_prj, float* d_vol, int view_begin,
+                int view_stride, int view_count, int bp_kind, void* stream);
+
+/* OS-SART, iterativeRecon/os_sart.m:1-44, on the device.  d_x (in/out) is the
+ * volume iterate, d_prj the full measured sinogram [PN][DNU][DNV].  Subset s holds
+ * views s, s+osNum, ...  resid (host, may be NULL) receives iterNum values: the
+ * relative data residual ||b - A x||/||b|| accumulated (FP64) over each sweep.
+ * Synchronous. */
+int ctr_os_sart(ctr_plan* plan, float* d_x, const float* d_prj, int osNum, int iterNum,
+                int bp_kind, double* resid);
+
+/* CG, iterativeRecon/conjgrad.m:1-35, on the device: A = Back(Proj(.)),
+ * b = Back(prj); dot products and step scalars in FP64.  resid (host, may be
+ * NULL) receives ||r_k|| per iteration.  Synchronous. */
+int ctr_conjgrad(ctr_plan* plan, float* d_x, const float* d_prj, int iterNum, int bp_kind,
+                 double* resid);
+
+/* Relative data residual ||A x - b|| / ||b|| over all views (FP64 sums). Synchronous. */
+int ctr_residual(ctr_plan* plan, const float* d_x, const float* d_prj, double* out);
+
+/* ---- device memory helpers (so that hosts need no other GPU runtime) ---- */
+int ctr_device_count(int* count);
+int ctr_malloc(void** d_ptr, unsigned long long bytes, int device);
+int ctr_free(void* d_ptr);
+int ctr_memcpy_h2d(void* d_dst, const void* h_src, unsigned long long bytes);
+int ctr_memcpy_d2h(void* h_dst, const void* d_src, unsigned long long bytes);
+int ctr_memset(void* d_dst, int value, unsigned long long bytes);
+int ctr_synchronize(int device);
+
+/* Message for the last failing call on this thread ("" if none). */
+const char* ctr_last_error(void);
+/* Library version string. */
+const char* ctr_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CTRECON_H */
