@@ -1,0 +1,216 @@
+"""Benchmark: time to relative data residual 1e-3 of helical OS-SART at 512x512x256.
+
+One step is one OS-SART sweep (20 subsets; every view forward- and back-projected
+once) over the clinical-size helical scan: 512x512x256 volume at 0.7 mm, 888x64
+detector, 984 views/turn x 4 turns (3936 views), pitch 0.9, FOV mask r^2 < 0.99.
+The measured sinogram b = A x_true (FP32, noise-free) of a seeded 40-ellipsoid
+phantom is resident in HBM before timing starts; x_0 = 0.  The timed solve
+includes the row/column-sum set-up.  ``value`` is the wall time until the relative
+data residual ||b - A x|| / ||b|| (FP64 sums, accumulated during each sweep) first
+reaches the target.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+
+    python bench.py [--steps K] [--warmup W] [--target 1e-3] [--no-cpu]
+
+--steps 0 (default) runs sweeps until the target is met (at most --max-steps);
+--steps K runs exactly K sweeps and reads the time-to-target off the per-sweep
+HIP-event times.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = ("time to rel. residual 1e-3, 512²×256 helical OS-SART: learned+BF16 vs "
+          "plain FP32")
+HBM_PEAK = 8.0e12  # MI355X, bytes/s
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(g, vol, n_views, target_sweeps, setup_sweeps=1.0):
+    """The oracle's FP32 restatement of the reference path (integral images + bilinear
+    samples, OpenMP over the host cores) on a bounded sample of the same scan."""
+    from oracle import oracle as O
+
+    views = np.linspace(0, g.PN - 1, n_views).round().astype(np.int64)
+    gs = g.subset(views)
+    t0 = time.perf_counter()
+    prj = O.fp_f32(gs, vol)
+    t1 = time.perf_counter()
+    O.bp_f32(gs, prj)
+    t2 = time.perf_counter()
+    per_view = (t2 - t0) / n_views  # one forward + one back projection of one view
+    out = {
+        "unit": "s",
+        "cores": O.num_threads(),
+        "kind": "port",
+        "projections_per_s": 1.0 / per_view,
+        "fp_s": t1 - t0,
+        "bp_s": t2 - t1,
+        "sample": (f"{n_views} views spread over the 3936-view scan, FP32 forward + back "
+                   f"projection of the full 512x512x256 volume ({t2 - t0:.1f} s); value = "
+                   f"(sweeps to target + 1 set-up sweep) x 3936 views x measured s/view"),
+    }
+    out["value"] = (None if target_sweeps is None else
+                    (target_sweeps + setup_sweeps) * g.PN * per_view)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=0)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--target", type=float, default=1e-3)
+    ap.add_argument("--max-steps", type=int, default=400)
+    ap.add_argument("--subsets", type=int, default=20)
+    ap.add_argument("--bp", default="ref", choices=["ref", "matched"])
+    ap.add_argument("--cpu-views", type=int, default=24)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="only run the plain copy kernel (byte-counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        log("bench.py measures one GPU: this build has no multi-GPU path (one GPU holds the "
+            "whole volume and sinogram); run with --gpus 1.")
+        sys.exit(2)
+
+    from ctrecon_amd import geometry as G
+    from ctrecon_amd import phantom as P
+    from ctrecon_amd._lib import lib
+    from ctrecon_amd.plan import DeviceArray, Plan, device_count
+
+    if device_count() < 1:
+        log("no GPU visible")
+        sys.exit(1)
+
+    if args.calib_copy:
+        n = 1 << 26  # 64 Mi floats = 256 MiB read + 256 MiB written per launch
+        a, b = DeviceArray((n,)), DeviceArray((n,))
+        a.zero()
+        for _ in range(5):
+            lib().ctr_device_copy(b.ptr, a.ptr, n, None)
+        lib().ctr_synchronize(0)
+        print(json.dumps({"calib_copy_bytes_read": 4 * n, "calib_copy_bytes_written": 4 * n,
+                          "launches": 5}))
+        return
+
+    cfg = G.config_clinical()
+    g = G.dd3_geometry(cfg, mask=G.fov_mask(512))
+    log("building phantom")
+    x_true = P.ellipsoid_mixture(g, seed=1)
+    plan = Plan(g, 0)
+    d_true = DeviceArray.from_host(x_true)
+    d_b = DeviceArray((g.PN, g.DNU, g.DNV))
+    plan.fp(d_true, d_b)
+    plan.synchronize()
+    d_x = DeviceArray(g.vol_shape)
+
+    if args.warmup > 0:
+        d_x.zero()
+        plan.os_sart_run(d_x, d_b, args.subsets, args.warmup, args.bp)
+        log(f"warmup: {args.warmup} sweep(s) done")
+
+    d_x.zero()
+    plan.synchronize()
+    max_iter = args.steps if args.steps > 0 else args.max_steps
+    stop = 0.0 if args.steps > 0 else args.target
+    t0 = time.perf_counter()
+    res, sweep_ms, st = plan.os_sart_run(d_x, d_b, args.subsets, max_iter, args.bp, stop)
+    plan.synchronize()
+    wall = time.perf_counter() - t0
+    steps = len(res)
+    cum = st["setup_ms"] + np.cumsum(sweep_ms)
+    hit = np.nonzero(res <= args.target)[0]
+    t_target = float(cum[hit[0]]) / 1e3 if hit.size else None
+    sweeps_target = int(hit[0]) + 1 if hit.size else None
+    log(f"{steps} sweeps, wall {wall:.2f}s, residual {res[-1]:.3e}, "
+        f"time to target {t_target}")
+
+    true_res = plan.residual(d_x, d_b)
+    x = d_x.get()
+    m = g.mask.reshape(g.YN, g.XN).astype(bool)
+    rmse = float(np.sqrt(np.mean((x - x_true)[m] ** 2)))
+
+    # roofline of the dominant kernel (the back-projection update)
+    fp_avg = st["fp_ms"] / max(st["fp_launches"], 1)
+    bp_avg = st["bp_ms"] / max(st["bp_launches"], 1)
+    cone = plan.cone_voxels()
+    frame = g.DNU * g.DNV
+    bytes_per_view = 4.0 * cone + 4.0 * frame
+    per_launch_bytes = float(bytes_per_view.sum()) / args.subsets
+    views_per_launch = g.PN / args.subsets
+    dominant = "bp_ref_kernel" if args.bp == "ref" else "bp_matched_kernel"
+    dom_ms, other_ms = (bp_avg, fp_avg)
+    achieved = per_launch_bytes / (dom_ms / 1e3)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get(dominant + "<true>")
+    out = {
+        "metric": METRIC,
+        "value": t_target,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": float(np.mean(sweep_ms)) if steps else None,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded 40-ellipsoid phantom (seed 1) + uniform texture; "
+                "b = A x_true (noise-free FP32)",
+        "config": {
+            "workload": "512x512x256 helical OS-SART, plain FP32, reference back projector"
+            if args.bp == "ref" else "512x512x256 helical OS-SART, plain FP32, matched adjoint",
+            "form": "plain FP32 (the learned + BF16 form is not built yet)",
+            "volume": [512, 512, 256], "voxel_mm": 0.7, "detector": [888, 64],
+            "views": int(g.PN), "views_per_turn": 984, "pitch": 0.9, "subsets": args.subsets,
+            "target_rel_residual": args.target,
+        },
+        "roofline": {
+            "kernel": dominant,
+            "bound": "hbm",
+            "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": per_launch_bytes,
+            "avg_launch_ms": dom_ms,
+        },
+        "sweeps_to_target": sweeps_target,
+        "setup_ms": st["setup_ms"],
+        "residual_sweep_final": float(res[-1]) if steps else None,
+        "residual_true_final": true_res,
+        "rmse_vs_truth_in_mask": rmse,
+        "projections_per_s": {
+            "os_sart_loop": g.PN / (np.mean(sweep_ms) / 1e3) if steps else None,
+            "fp_kernel": views_per_launch / (fp_avg / 1e3),
+            "bp_kernel": views_per_launch / (bp_avg / 1e3),
+        },
+        "kernel_avg_ms": {"fp_fused": fp_avg, "bp_update": bp_avg},
+    }
+    if not args.no_cpu:
+        log("cpu baseline sample")
+        out["cpu_baseline"] = cpu_baseline(g, x_true, args.cpu_views, sweeps_target)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -27,9 +27,10 @@ _NAMES = {0: "CTR_OK", 1: "CTR_ERR_ARG", 2: "CTR_ERR_HIP", 3: "CTR_ERR_UNSUPPORT
 # Every symbol include/ctrecon.h declares.
 EXPORTS = [
     "ctr_dd3_proj", "ctr_dd3_back", "ctr_plan_create", "ctr_plan_destroy", "ctr_plan_fp",
-    "ctr_plan_bp", "ctr_os_sart", "ctr_conjgrad", "ctr_residual", "ctr_device_count",
+    "ctr_plan_bp", "ctr_os_sart", "ctr_os_sart_run", "ctr_cone_voxels", "ctr_conjgrad",
+    "ctr_residual", "ctr_device_count",
     "ctr_malloc", "ctr_free", "ctr_memcpy_h2d", "ctr_memcpy_d2h", "ctr_memset",
-    "ctr_synchronize", "ctr_last_error", "ctr_version",
+    "ctr_synchronize", "ctr_device_copy", "ctr_last_error", "ctr_version",
 ]
 
 
@@ -53,6 +54,13 @@ class CtrGeometry(ctypes.Structure):
     ]
 
 
+class CtrSartStats(ctypes.Structure):
+    _fields_ = [
+        ("sweeps", ctypes.c_int), ("fp_launches", ctypes.c_int), ("bp_launches", ctypes.c_int),
+        ("setup_ms", ctypes.c_double), ("fp_ms", ctypes.c_double), ("bp_ms", ctypes.c_double),
+    ]
+
+
 _lib = None
 
 
@@ -72,6 +80,8 @@ def lib():
         L.ctr_plan_fp.argtypes = [p, p, p, i, i, i, p]
         L.ctr_plan_bp.argtypes = [p, p, p, i, i, i, i, p]
         L.ctr_os_sart.argtypes = [p, p, p, i, i, i, p]
+        L.ctr_os_sart_run.argtypes = [p, p, p, i, i, i, d, p, p, ctypes.POINTER(CtrSartStats)]
+        L.ctr_cone_voxels.argtypes = [p, i, i, i, p]
         L.ctr_conjgrad.argtypes = [p, p, p, i, i, p]
         L.ctr_residual.argtypes = [p, p, p, ctypes.POINTER(d)]
         L.ctr_device_count.argtypes = [ctypes.POINTER(i)]
@@ -81,6 +91,7 @@ def lib():
         L.ctr_memcpy_d2h.argtypes = [p, p, ctypes.c_ulonglong]
         L.ctr_memset.argtypes = [p, i, ctypes.c_ulonglong]
         L.ctr_synchronize.argtypes = [i]
+        L.ctr_device_copy.argtypes = [p, p, ctypes.c_ulonglong, p]
         L.ctr_last_error.restype = ctypes.c_char_p
         L.ctr_version.restype = ctypes.c_char_p
         for name in EXPORTS:

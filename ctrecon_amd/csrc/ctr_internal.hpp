@@ -36,6 +36,13 @@ struct BpConsts {
   float dx, dz;
 };
 
+// FP64 copies of what the reference back projector needs for the in-plane
+// (fan-angle) coordinate, which must be resolved to well below a detector cell
+// on detectors several hundred cells wide.
+struct BpConsts64 {
+  double ib, cU, inv_sid, dx, ocx, ocy;
+};
+
 // Geometry constants every kernel needs.
 struct Dims {
   int XN, YN, ZN, DNU, DNV, PN;
@@ -45,12 +52,21 @@ struct Dims {
   float B0, dB;          // row boundary index of row 0 and mean spacing (adjoint guess)
 };
 
+// Host copies used by host-side accounting (cone voxel counts).
+struct HostGeo {
+  std::vector<double> sz;     // source z per view
+  std::vector<uint8_t> mask;  // 1 where the mask byte is non-zero
+  double ocx, ocy, ocz, dx, dz, S2D, bz0, bzN;
+};
+
 struct Plan {
   int device = 0;
   Dims d{};
   BpConsts bc{};
+  BpConsts64 bc64{};
   // device tables
   ViewP* views = nullptr;      // [PN]
+  double2* src64 = nullptr;    // [PN]         rotated source (x, y) in FP64
   float2* bnd = nullptr;       // [PN][DNU+1]  (a_b, f_b): t_b(ii) = a_b + ii*f_b
   float4* cols = nullptr;      // [PN][DNU]    (r0, r1, G, Q)
   float* Bz = nullptr;         // [DNV+1]      (bz_v - z0)/dz
@@ -59,6 +75,7 @@ struct Plan {
   uint8_t* mask = nullptr;     // [XN*YN]
   // host copies (for culling and solvers)
   std::vector<ViewP> h_views;
+  HostGeo hg;
   // scratch
   float* scratch_prj = nullptr;     // [max_views][DNU][DNV]
   size_t scratch_prj_views = 0;
@@ -85,6 +102,7 @@ int launch_bp(const Plan& P, const float* prj, float* vol, int vb, int vs, int v
 int launch_scale_matched(const Plan& P, const float* in, float* out, int vb, int vs, int vc,
                          hipStream_t st);
 int launch_fill(float* p, size_t n, float v, hipStream_t st);
+int launch_copy(float* dst, const float* src, size_t n, hipStream_t st);
 int launch_inv_guard(const float* in, float* out, size_t n, hipStream_t st);
 int launch_axpy(float* y, const float* x, double a, size_t n, hipStream_t st);
 int launch_xpby(float* p, const float* r, double b, size_t n, hipStream_t st);

@@ -144,8 +144,8 @@ __global__ __launch_bounds__(kBlock) void fp_kernel(
 template <bool UPDATE>
 __global__ __launch_bounds__(kBlock) void bp_ref_kernel(
     const float* __restrict__ prj, float* __restrict__ vol, const ViewP* __restrict__ views,
-    const uint8_t* __restrict__ mask, Dims d, BpConsts c, int vb, int vs, int vc,
-    const float* __restrict__ invCol) {
+    const double2* __restrict__ src64, const uint8_t* __restrict__ mask, Dims d, BpConsts c,
+    BpConsts64 c64, int vb, int vs, int vc, const float* __restrict__ invCol) {
   const int lane = threadIdx.x & 63;
   const int col = blockIdx.x * kWaves + (threadIdx.x >> 6);
   const int z = blockIdx.y * 64 + lane;
@@ -156,44 +156,54 @@ __global__ __launch_bounds__(kBlock) void bp_ref_kernel(
     if constexpr (!UPDATE) vol[o] = 0.f;
     return;
   }
-  const float Px = ((float)x - d.ocx) * d.dx;
-  const float Py = ((float)y - d.ocy) * d.dx;
+  // voxel centre; the in-plane part in FP64 (same for the whole wave)
+  const double Px = ((double)x - c64.ocx) * c64.dx;
+  const double Py = ((double)y - c64.ocy) * c64.dx;
   const float Pz = ((float)z - d.ocz) * d.dz;
-  const float DNUh = (float)d.DNU - 0.5f, DNVh = (float)d.DNV - 0.5f;
+  const double DNUh = (double)d.DNU - 0.5;
+  const float DNVh = (float)d.DNV - 0.5f;
   const size_t frame = (size_t)d.DNU * d.DNV;
   float acc = 0.f;
   for (int k = 0; k < vc; ++k) {
-    const ViewP vp = views[vb + k * vs];
-    const float dX = Px - vp.sx, dY = Py - vp.sy, dZ = Pz - vp.sz;
-    const float l2 = dX * dX + dY * dY;
-    const float rl = rsqrtf(l2);
-    const float tU = (dZ + c.hz) * c.S * rl + c.cV;
-    const float tD = (dZ - c.hz) * c.S * rl + c.cV;
+    const int p = vb + k * vs;
+    const ViewP vp = views[p];
+    const double2 s64 = src64[p];
+    const double dX = Px - s64.x, dY = Py - s64.y;
+    const double l2 = dX * dX + dY * dY;
+    const double rl = rsqrt(l2);
+    const float rlf = (float)rl;
+    // detector-row shadow, per lane (DD3_GPU.cu:1449-1450)
+    const float dZ = Pz - vp.sz;
+    const float tU = (dZ + c.hz) * c.S * rlf + c.cV;
+    const float tD = (dZ - c.hz) * c.S * rlf + c.cV;
     const float vlo = fmaxf(fminf(tU, tD), -0.5f);
     const float vhi = fminf(fmaxf(tU, tD), DNVh);
     const bool vok = vlo < vhi;
     if (!__any(vok)) continue;
-    const float alpha = asinf((vp.sy * dX - vp.sx * dY) * c.inv_sid * rl);
-    const float w = vp.wx ? dX : dY;
-    const float dA = w / l2 * c.dx * 0.5f;
-    const float cosVal = c.dx / w * sqrtf(l2 + dZ * dZ);
-    const float tL = (alpha - dA) * c.ib + c.cU;
-    const float tR = (alpha + dA) * c.ib + c.cU;
-    const float ulo = fmaxf(fminf(tL, tR), -0.5f);
-    const float uhi = fminf(fmaxf(tL, tR), DNUh);
+    // fan-angle shadow (DD3_GPU.cu:1451-1464); sy*dX - sx*dY == sy*Px - sx*Py
+    const double alpha = asin((s64.y * Px - s64.x * Py) * c64.inv_sid * rl);
+    const double w = vp.wx ? dX : dY;
+    const double dA = w / l2 * c64.dx * 0.5;
+    const double tL = (alpha - dA) * c64.ib + c64.cU;
+    const double tR = (alpha + dA) * c64.ib + c64.cU;
+    const double ulo = fmax(fmin(tL, tR), -0.5);
+    const double uhi = fmin(fmax(tL, tR), DNUh);
     if (!(ulo < uhi) || !vok) continue;
-    const float sgn = ((tR - tL) * (tU - tD) < 0.f) ? -1.f : 1.f;
-    const int u0 = (int)floorf(ulo + 0.5f);
-    const int u1 = min((int)floorf(uhi + 0.5f), d.DNU - 1);
+    const float wf = (float)w;
+    const float cosVal = c.dx / wf * sqrtf((float)l2 + dZ * dZ);
+    const float sgn = ((tR - tL) * (double)(tU - tD) < 0.0) ? -1.f : 1.f;
+    const int u0 = (int)floor(ulo + 0.5);
+    const int u1 = min((int)floor(uhi + 0.5), d.DNU - 1);
     const int v0 = (int)floorf(vlo + 0.5f);
     const int v1 = min((int)floorf(vhi + 0.5f), d.DNV - 1);
     const float* pp = prj + (size_t)k * frame;
     float s = 0.f;
     for (int u = u0; u <= u1; ++u) {
+      const float wu = (float)(fmin(uhi, (double)u + 0.5) - fmax(ulo, (double)u - 0.5));
       const float* row = pp + (size_t)u * d.DNV;
       float g = 0.f;
       for (int v = v0; v <= v1; ++v) g = fmaf(cell_w(vlo, vhi, (float)v), row[v], g);
-      s = fmaf(cell_w(ulo, uhi, (float)u), g, s);
+      s = fmaf(wu, g, s);
     }
     acc = fmaf(sgn * cosVal, s, acc);
   }
@@ -323,6 +333,12 @@ __global__ __launch_bounds__(kBlock) void scale_matched_kernel(
   out[o] = in[o] * K;
 }
 
+__global__ void copy_kernel(float* __restrict__ dst, const float* __restrict__ src, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
 __global__ void fill_kernel(float* __restrict__ p, size_t n, float v) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
@@ -429,11 +445,11 @@ int launch_bp(const Plan& P, const float* prj, float* vol, int vb, int vs, int v
     return check_launch("bp_matched_kernel");
   }
   if (accumulate)
-    hipLaunchKernelGGL(bp_ref_kernel<true>, grid, dim3(kBlock), 0, st, prj, vol, P.views, P.mask,
-                       P.d, P.bc, vb, vs, vc, invCol);
+    hipLaunchKernelGGL(bp_ref_kernel<true>, grid, dim3(kBlock), 0, st, prj, vol, P.views, P.src64,
+                       P.mask, P.d, P.bc, P.bc64, vb, vs, vc, invCol);
   else
-    hipLaunchKernelGGL(bp_ref_kernel<false>, grid, dim3(kBlock), 0, st, prj, vol, P.views, P.mask,
-                       P.d, P.bc, vb, vs, vc, nullptr);
+    hipLaunchKernelGGL(bp_ref_kernel<false>, grid, dim3(kBlock), 0, st, prj, vol, P.views,
+                       P.src64, P.mask, P.d, P.bc, P.bc64, vb, vs, vc, nullptr);
   return check_launch("bp_ref_kernel");
 }
 
@@ -444,6 +460,11 @@ int launch_scale_matched(const Plan& P, const float* in, float* out, int vb, int
   hipLaunchKernelGGL(scale_matched_kernel, grid, dim3(kBlock), 0, st, in, out, P.cols, P.h2,
                      P.invdB, P.d, vb, vs);
   return check_launch("scale_matched_kernel");
+}
+
+int launch_copy(float* dst, const float* src, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(copy_kernel, dim3(grid_1d(n)), dim3(256), 0, st, dst, src, n);
+  return check_launch("copy_kernel");
 }
 
 int launch_fill(float* p, size_t n, float v, hipStream_t st) {

@@ -148,10 +148,17 @@ static int build_tables(const ctr_geometry* g, Plan& P) {
     P.bc.cV = (float)cV;
     P.bc.dx = (float)dx;
     P.bc.dz = (float)dz;
+    P.bc64.ib = 1.0 / dbeta;
+    P.bc64.cU = cU;
+    P.bc64.inv_sid = 1.0 / sid;
+    P.bc64.dx = dx;
+    P.bc64.ocx = ocx;
+    P.bc64.ocy = ocy;
   }
 
   // per-view tables
   std::vector<ViewP> views(PN);
+  std::vector<double2> src64(PN);
   std::vector<float2> bnd((size_t)PN * (DNU + 1));
   std::vector<float4> cols((size_t)PN * DNU);
   for (int p = 0; p < PN; ++p) {
@@ -160,6 +167,7 @@ static int build_tables(const ctr_geometry* g, Plan& P) {
     const double sx = x0 * c - y0 * s, sy = x0 * s + y0 * c, sz = z0 + g->zshifts[p];
     ViewP& vp = views[p];
     vp.sx = (float)sx; vp.sy = (float)sy; vp.sz = (float)sz;
+    src64[p] = make_double2(sx, sy);
     vp.az = (float)(sz / dz + ocz);
     vp.xmarch = std::fabs(c) <= std::fabs(s) ? 1 : 0;  // DD3_GPU.cu:914
     vp.wx = std::fabs(sx) > std::fabs(sy) ? 1 : 0;  // DD3_GPU.cu:1453
@@ -201,12 +209,25 @@ static int build_tables(const ctr_geometry* g, Plan& P) {
     }
   }
   P.h_views = views;
+  {
+    HostGeo& h = P.hg;
+    h.sz.resize(PN);
+    for (int p = 0; p < PN; ++p) h.sz[p] = z0 + g->zshifts[p];
+    h.mask.assign((size_t)g->XN * g->YN, 1);
+    if (g->mask)
+      for (size_t i = 0; i < h.mask.size(); ++i) h.mask[i] = g->mask[i] != 0;
+    h.ocx = ocx; h.ocy = ocy; h.ocz = ocz; h.dx = dx; h.dz = dz;
+    h.S2D = std::hypot((double)g->xds[0] - x0, (double)g->yds[0] - y0);
+    h.bz0 = bz[0] - z0;
+    h.bzN = bz[DNV] - z0;
+  }
 
   std::vector<uint8_t> mask((size_t)g->XN * g->YN, 1);
   if (g->mask) std::memcpy(mask.data(), g->mask, mask.size());
 
   int rc;
   if ((rc = upload(&P.views, views))) return rc;
+  if ((rc = upload(&P.src64, src64))) return rc;
   if ((rc = upload(&P.bnd, bnd))) return rc;
   if ((rc = upload(&P.cols, cols))) return rc;
   if ((rc = upload(&P.Bz, Bz))) return rc;
@@ -219,6 +240,7 @@ static int build_tables(const ctr_geometry* g, Plan& P) {
 static void free_plan(Plan* P) {
   if (!P) return;
   (void)hipFree(P->views);
+  (void)hipFree(P->src64);
   (void)hipFree(P->bnd);
   (void)hipFree(P->cols);
   (void)hipFree(P->Bz);
@@ -270,6 +292,19 @@ static int bp_any(Plan& P, const float* prj, float* vol, int vb, int vs, int vc,
   }
   return launch_bp(P, prj, vol, vb, vs, vc, kind, st, invCol, accumulate);
 }
+
+// Owns a set of HIP events.
+struct EventPool {
+  std::vector<hipEvent_t> e;
+  ~EventPool() {
+    for (auto x : e) (void)hipEventDestroy(x);
+  }
+  int init(int n) {
+    e.resize(n, nullptr);
+    for (auto& x : e) CTR_HIP(hipEventCreate(&x));
+    return CTR_OK;
+  }
+};
 
 // RAII device buffer used by the drivers.
 struct DBuf {
@@ -430,9 +465,15 @@ int ctr_residual(ctr_plan* plan, const float* d_x, const float* d_prj, double* o
 
 int ctr_os_sart(ctr_plan* plan, float* d_x, const float* d_prj, int osNum, int iterNum,
                 int bp_kind, double* resid) {
+  return ctr_os_sart_run(plan, d_x, d_prj, osNum, iterNum, bp_kind, 0.0, resid, nullptr, nullptr);
+}
+
+int ctr_os_sart_run(ctr_plan* plan, float* d_x, const float* d_prj, int osNum, int maxIter,
+                    int bp_kind, double stop_resid, double* resid, double* sweep_ms,
+                    ctr_sart_stats* stats) {
   Plan* P = get(plan);
   if (!P || !d_x || !d_prj) return fail(CTR_ERR_ARG, "NULL argument");
-  if (osNum < 1 || osNum > P->d.PN || iterNum < 0)
+  if (osNum < 1 || osNum > P->d.PN || maxIter < 0)
     return fail(CTR_ERR_ARG, "osNum must be in [1, PN] and iterNum >= 0");
   if (bp_kind != CTR_BP_REFERENCE && bp_kind != CTR_BP_MATCHED)
     return fail(CTR_ERR_ARG, "bad bp_kind");
@@ -449,6 +490,13 @@ int ctr_os_sart(ctr_plan* plan, float* d_x, const float* d_prj, int osNum, int i
   if ((rc = sums.alloc(2 * sizeof(double) * osNum))) return rc;
   if ((rc = ensure_partials(*P, 2 * (size_t)fp_partial_blocks(*P, maxV) + 2))) return rc;
   hipStream_t st = nullptr;
+  const bool timing = stats != nullptr || sweep_ms != nullptr;
+  // events: [0] setup start, [1] setup end / sweep 0 start, then one per sweep end;
+  // per-launch pairs for the kernel totals
+  EventPool ev;
+  if (timing && (rc = ev.init(2 + maxIter + 4 * osNum))) return rc;
+  if (stats) std::memset(stats, 0, sizeof(*stats));
+  if (timing) CTR_HIP(hipEventRecord(ev.e[0], st));
   // RowSum{s} = fp(ones), ColSum{s} = bp(ones) (os_sart.m:14-18)
   if ((rc = launch_fill(ones.f(), std::max(NV, frame * maxV), 1.0f, st))) return rc;
   std::vector<size_t> rowOff(osNum);
@@ -464,19 +512,27 @@ int ctr_os_sart(ctr_plan* plan, float* d_x, const float* d_prj, int osNum, int i
     if ((rc = launch_inv_guard(cI, cI, NV, st))) return rc;
     off += vc;
   }
-  for (int it = 0; it < iterNum; ++it) {
+  if (timing) CTR_HIP(hipEventRecord(ev.e[1], st));
+  hipEvent_t* kev = timing ? ev.e.data() + 2 + maxIter : nullptr;
+  int done = 0;
+  for (int it = 0; it < maxIter; ++it) {
     for (int s = 0; s < osNum; ++s) {
       const int vc = (d.PN - s + osNum - 1) / osNum;
+      if (stats) CTR_HIP(hipEventRecord(kev[4 * s + 0], st));
       // tp = (prj - A_s x) ./ RowSum (os_sart.m:24,30-31)
       if ((rc = launch_fp(*P, d_x, tp.f(), s, osNum, vc, st, d_prj, rowInv.f() + rowOff[s] * frame,
                           P->partials)))
         return rc;
+      if (stats) CTR_HIP(hipEventRecord(kev[4 * s + 1], st));
       if ((rc = launch_reduce_pairs(P->partials, fp_partial_blocks(*P, vc), sums.d() + 2 * s, st)))
         return rc;
+      if (stats) CTR_HIP(hipEventRecord(kev[4 * s + 2], st));
       // x += B_s(tp) ./ ColSum (os_sart.m:32,37-40)
       if ((rc = bp_any(*P, tp.f(), d_x, s, osNum, vc, bp_kind, st, colInv.f() + NV * s, 1)))
         return rc;
+      if (stats) CTR_HIP(hipEventRecord(kev[4 * s + 3], st));
     }
+    if (timing) CTR_HIP(hipEventRecord(ev.e[2 + it], st));
     std::vector<double> h(2 * osNum);
     CTR_HIP(hipMemcpy(h.data(), sums.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
     double rr = 0, bb = 0;
@@ -484,9 +540,63 @@ int ctr_os_sart(ctr_plan* plan, float* d_x, const float* d_prj, int osNum, int i
       rr += h[2 * s];
       bb += h[2 * s + 1];
     }
-    if (resid) resid[it] = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+    const double r = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+    if (resid) resid[it] = r;
+    if (stats) {
+      for (int s = 0; s < osNum; ++s) {
+        float a = 0, b = 0;
+        CTR_HIP(hipEventElapsedTime(&a, kev[4 * s + 0], kev[4 * s + 1]));
+        CTR_HIP(hipEventElapsedTime(&b, kev[4 * s + 2], kev[4 * s + 3]));
+        stats->fp_ms += a;
+        stats->bp_ms += b;
+      }
+      stats->fp_launches += osNum;
+      stats->bp_launches += osNum;
+    }
+    if (sweep_ms) {
+      float ms = 0;
+      CTR_HIP(hipEventElapsedTime(&ms, ev.e[1 + it], ev.e[2 + it]));
+      sweep_ms[it] = ms;
+    }
+    done = it + 1;
+    if (stop_resid > 0 && r <= stop_resid) break;
   }
   CTR_HIP(hipDeviceSynchronize());
+  if (stats) {
+    float ms = 0;
+    CTR_HIP(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
+    stats->setup_ms = ms;
+    stats->sweeps = done;
+  }
+  return CTR_OK;
+}
+
+int ctr_cone_voxels(ctr_plan* plan, int view_begin, int view_stride, int view_count,
+                    long long* out) {
+  Plan* P = get(plan);
+  if (!P || !out) return fail(CTR_ERR_ARG, "NULL argument");
+  int rc = check_views(*P, view_begin, view_stride, view_count);
+  if (rc) return rc;
+  const HostGeo& h = P->hg;
+  const Dims& d = P->d;
+  for (int k = 0; k < view_count; ++k) {
+    const int p = view_begin + k * view_stride;
+    const double sx = P->h_views[p].sx, sy = P->h_views[p].sy, sz = h.sz[p];
+    long long n = 0;
+    for (int y = 0; y < d.YN; ++y)
+      for (int x = 0; x < d.XN; ++x) {
+        if (!h.mask[(size_t)y * d.XN + x]) continue;
+        const double Px = (x - h.ocx) * h.dx - sx, Py = (y - h.ocy) * h.dx - sy;
+        const double l = std::sqrt(Px * Px + Py * Py) / h.S2D;
+        const double zlo = sz + h.bz0 * l, zhi = sz + h.bzN * l;
+        int k0 = (int)std::floor(zlo / h.dz + h.ocz - 0.5) + 1;
+        int k1 = (int)std::ceil(zhi / h.dz + h.ocz + 0.5) - 1;
+        k0 = std::max(k0, 0);
+        k1 = std::min(k1, d.ZN - 1);
+        if (k1 >= k0) n += k1 - k0 + 1;
+      }
+    out[k] = n;
+  }
   return CTR_OK;
 }
 
@@ -638,6 +748,11 @@ int ctr_memset(void* d_dst, int value, unsigned long long bytes) {
   if (!d_dst) return fail(CTR_ERR_ARG, "NULL pointer");
   CTR_HIP(hipMemset(d_dst, value, bytes));
   return CTR_OK;
+}
+
+int ctr_device_copy(float* d_dst, const float* d_src, unsigned long long n, void* stream) {
+  if (!d_dst || !d_src) return fail(CTR_ERR_ARG, "NULL pointer");
+  return launch_copy(d_dst, d_src, n, (hipStream_t)stream);
 }
 
 int ctr_synchronize(int device) {

@@ -13,7 +13,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import CTR_BP_MATCHED, CTR_BP_REFERENCE, CtrGeometry, check, lib
+from ._lib import CTR_BP_MATCHED, CTR_BP_REFERENCE, CtrGeometry, CtrSartStats, check, lib
 from .geometry import Geometry
 
 __all__ = ["DeviceArray", "Plan", "device_count", "bp_kind_code"]
@@ -129,6 +129,29 @@ class Plan:
         check(lib().ctr_os_sart(self.handle, d_x.ptr, d_prj.ptr, osNum, iterNum,
                                 bp_kind_code(kind), res.ctypes.data))
         return res[:iterNum]
+
+    def os_sart_run(self, d_x: DeviceArray, d_prj: DeviceArray, osNum: int, maxIter: int,
+                    kind="ref", stop_resid=0.0):
+        """OS-SART with a stopping rule and HIP-event timing; returns
+        (residuals, sweep_ms, stats) for the sweeps actually run."""
+        res = np.zeros(max(maxIter, 1), dtype=np.float64)
+        ms = np.zeros(max(maxIter, 1), dtype=np.float64)
+        st = CtrSartStats()
+        check(lib().ctr_os_sart_run(self.handle, d_x.ptr, d_prj.ptr, osNum, maxIter,
+                                    bp_kind_code(kind), float(stop_resid), res.ctypes.data,
+                                    ms.ctypes.data, ctypes.byref(st)))
+        n = st.sweeps
+        stats = {k: getattr(st, k) for k, _ in CtrSartStats._fields_}
+        return res[:n], ms[:n], stats
+
+    def cone_voxels(self, view_begin=0, view_stride=1, view_count=None):
+        """Masked voxels inside each view's cone (host computation)."""
+        if view_count is None:
+            view_count = len(range(view_begin, self.g.PN, view_stride))
+        out = np.zeros(view_count, dtype=np.int64)
+        check(lib().ctr_cone_voxels(self.handle, view_begin, view_stride, view_count,
+                                    out.ctypes.data))
+        return out
 
     def conjgrad(self, d_x: DeviceArray, d_prj: DeviceArray, iterNum: int, kind="ref"):
         res = np.zeros(max(iterNum, 1), dtype=np.float64)

@@ -111,6 +111,32 @@ int ctr_plan_bp(ctr_plan* plan, const float* d_prj, float* d_vol, int view_begin
 int ctr_os_sart(ctr_plan* plan, float* d_x, const float* d_prj, int osNum, int iterNum,
                 int bp_kind, double* resid);
 
+/* Timing of ctr_os_sart_run (HIP events on the launch stream). */
+typedef struct ctr_sart_stats {
+  int sweeps;          /* sweeps run */
+  int fp_launches;     /* fused forward-projection launches timed */
+  int bp_launches;     /* back-projection (update) launches timed */
+  double setup_ms;     /* row / column sums and their guarded inverses */
+  double fp_ms;        /* summed duration of the forward-projection launches */
+  double bp_ms;        /* summed duration of the back-projection launches */
+} ctr_sart_stats;
+
+/* ctr_os_sart with a stopping rule and timing (no counterpart in os_sart.m, which
+ * has neither): runs at most maxIter sweeps, stops after the first sweep whose
+ * residual is <= stop_resid (stop_resid <= 0: never).  resid / sweep_ms (host,
+ * maxIter entries, may be NULL) receive the per-sweep residual and duration;
+ * stats (may be NULL) the set-up time and per-kernel totals.  Synchronous. */
+int ctr_os_sart_run(ctr_plan* plan, float* d_x, const float* d_prj, int osNum, int maxIter,
+                    int bp_kind, double stop_resid, double* resid, double* sweep_ms,
+                    ctr_sart_stats* stats);
+
+/* Number of masked voxels inside each view's cone (the voxels a view can see:
+ * z-cells meeting [sz + (bz_0 - z0) l/S2D, sz + (bz_DNV - z0) l/S2D] at in-plane
+ * source distance l), for views view_begin + k*view_stride.  Host computation;
+ * the per-view volume term of the algorithmic byte model. */
+int ctr_cone_voxels(ctr_plan* plan, int view_begin, int view_stride, int view_count,
+                    long long* out);
+
 /* CG, iterativeRecon/conjgrad.m:1-35, on the device: A = Back(Proj(.)),
  * b = Back(prj); dot products and step scalars in FP64.  resid (host, may be
  * NULL) receives ||r_k|| per iteration.  Synchronous. */
@@ -128,6 +154,8 @@ int ctr_memcpy_h2d(void* d_dst, const void* h_src, unsigned long long bytes);
 int ctr_memcpy_d2h(void* h_dst, const void* d_src, unsigned long long bytes);
 int ctr_memset(void* d_dst, int value, unsigned long long bytes);
 int ctr_synchronize(int device);
+/* Plain copy kernel of n floats (also the byte-counter calibration kernel). Asynchronous. */
+int ctr_device_copy(float* d_dst, const float* d_src, unsigned long long n, void* stream);
 
 /* Message for the last failing call on this thread ("" if none). */
 const char* ctr_last_error(void);
