@@ -12,6 +12,8 @@
 // the array, so the kernels integrate the footprint directly: the same operator,
 // without the FP32 cancellation of whole-plane integral images and without the
 // integral-image build pass.
+#include <algorithm>
+
 #include "ctr_internal.hpp"
 
 namespace ctr {
@@ -59,18 +61,29 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* partials,
 // invRow by the launch's compact view index, NULL for 1) and per-block sums of
 // (b - A x)^2 and b^2.
 // ---------------------------------------------------------------------------
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2):
+// remap the flat workgroup id so that every XCD works through one contiguous range
+// of logical blocks (neighbouring detector columns of the same view).
+__device__ __forceinline__ int xcd_logical_block(int hw, int G) {
+  const int q = G >> 3, r = G & 7, xcd = hw & 7, loc = hw >> 3;
+  return (xcd < r) ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
+}
+
 template <bool FUSED>
 __global__ __launch_bounds__(kBlock) void fp_kernel(
     const float* __restrict__ vol, float* __restrict__ out, const ViewP* __restrict__ views,
     const float2* __restrict__ bnd, const float4* __restrict__ cols,
     const float* __restrict__ Bz, const float* __restrict__ h2,
-    const float* __restrict__ invdB, const uint8_t* __restrict__ mask, Dims d, int vb, int vs,
+    const float* __restrict__ invdB, const uint8_t* __restrict__ mask,
+    const float2* __restrict__ mext, Dims d, int vb, int vs,
     const float* __restrict__ b, const float* __restrict__ invRow,
     double* __restrict__ partials) {
-  const int k = blockIdx.y;
+  const int hw = blockIdx.y * gridDim.x + blockIdx.x;
+  const int lb = xcd_logical_block(hw, gridDim.x * gridDim.y);
+  const int k = lb / gridDim.x;
   const int p = vb + k * vs;
   const int nray = d.DNU * d.DNV;
-  const int ray = blockIdx.x * kBlock + threadIdx.x;
+  const int ray = (lb - k * gridDim.x) * kBlock + threadIdx.x;
   double res = 0.0, bsq = 0.0;
   if (ray < nray) {
     const int u = ray / d.DNV;
@@ -86,7 +99,8 @@ __global__ __launch_bounds__(kBlock) void fp_kernel(
     } else {
       NS = d.YN; NI = d.XN; sS = d.XN * d.ZN; cS = d.ZN; mS = d.XN; mC = 1;
     }
-    const float NIh = (float)NI - 0.5f, ZNh = (float)d.ZN - 0.5f;
+    const float ZNh = (float)d.ZN - 0.5f;
+    const float2* me = mext + (vp.xmarch ? 0 : d.XN);
     float acc = 0.f;
     for (int ii = 0; ii < NS; ++ii) {
       const float fi = (float)ii;
@@ -95,8 +109,11 @@ __global__ __launch_bounds__(kBlock) void fp_kernel(
       const float r = fmaf(fi, ct.y, ct.x);
       float ylo = fminf(tl, tr), yhi = fmaxf(tl, tr);
       const float inv = 1.0f / ((yhi - ylo) * r);
-      ylo = fmaxf(ylo, -0.5f);
-      yhi = fminf(yhi, NIh);
+      // clip to the mask's extent on this slice (inside [-0.5, NI - 0.5]); cells
+      // outside it are zero
+      const float2 e = me[ii];
+      ylo = fmaxf(ylo, e.x);
+      yhi = fminf(yhi, e.y);
       if (!(ylo < yhi)) continue;
       float zl = fmaf(r, Bv, vp.az), zh = fmaf(r, Bv1, vp.az);
       zl = fmaxf(zl, -0.5f);
@@ -131,86 +148,124 @@ __global__ __launch_bounds__(kBlock) void fp_kernel(
       out[o] = val;
     }
   }
-  if constexpr (FUSED) block_sum2(res, bsq, partials, blockIdx.y * gridDim.x + blockIdx.x);
+  if constexpr (FUSED) block_sum2(res, bsq, partials, hw);
 }
 
 // ---------------------------------------------------------------------------
-// Reference back projector, one thread per voxel, views looped inside.
+// Reference back projector: one wave per (x, y) column, each lane owning ZG
+// voxels z = z0 + lane + 64 g (g < ZG), views looped inside.
 // Reference: DD3_gpu_back_ker<_BRANCHLESS>, iterativeRecon/DD3_GPU.cu:1397-1475.
-// The voxel's shadow in (fan-angle index, row index) space is integrated over
-// the piecewise-constant sinogram and weighted by cosVal.
+// The voxel's shadow in (fan-angle index, row index) space is integrated over the
+// piecewise-constant sinogram and weighted by cosVal.  The fan-angle coordinate is
+// the same for the whole column and is resolved in FP64 once per (column, view);
+// the row coordinates are per lane in FP32.  A view whose row shadow misses the
+// detector for every lane of a group is skipped for that group.
 // UPDATE: vol += acc * invCol (os_sart.m:32,37-40); otherwise vol = acc.
 // ---------------------------------------------------------------------------
-template <bool UPDATE>
+template <bool UPDATE, int ZG>
 __global__ __launch_bounds__(kBlock) void bp_ref_kernel(
     const float* __restrict__ prj, float* __restrict__ vol, const ViewP* __restrict__ views,
     const double2* __restrict__ src64, const uint8_t* __restrict__ mask, Dims d, BpConsts c,
     BpConsts64 c64, int vb, int vs, int vc, const float* __restrict__ invCol) {
   const int lane = threadIdx.x & 63;
   const int col = blockIdx.x * kWaves + (threadIdx.x >> 6);
-  const int z = blockIdx.y * 64 + lane;
-  if (col >= d.XN * d.YN || z >= d.ZN) return;
+  if (col >= d.XN * d.YN) return;
+  const int zb = blockIdx.y * (64 * ZG) + lane;
   const int y = col / d.XN, x = col - y * d.XN;
-  const size_t o = (size_t)col * d.ZN + z;
+  float* vcol = vol + (size_t)col * d.ZN;
   if (mask[col] != 1) {
-    if constexpr (!UPDATE) vol[o] = 0.f;
+    if constexpr (!UPDATE) {
+#pragma unroll
+      for (int g = 0; g < ZG; ++g)
+        if (zb + 64 * g < d.ZN) vcol[zb + 64 * g] = 0.f;
+    }
     return;
   }
-  // voxel centre; the in-plane part in FP64 (same for the whole wave)
   const double Px = ((double)x - c64.ocx) * c64.dx;
   const double Py = ((double)y - c64.ocy) * c64.dx;
-  const float Pz = ((float)z - d.ocz) * d.dz;
+  const float Pxf = (float)Px, Pyf = (float)Py;
+  float Pz[ZG], acc[ZG];
+  bool live[ZG];
+#pragma unroll
+  for (int g = 0; g < ZG; ++g) {
+    Pz[g] = ((float)(zb + 64 * g) - d.ocz) * d.dz;
+    acc[g] = 0.f;
+    live[g] = zb + 64 * g < d.ZN;
+  }
   const double DNUh = (double)d.DNU - 0.5;
   const float DNVh = (float)d.DNV - 0.5f;
   const size_t frame = (size_t)d.DNU * d.DNV;
-  float acc = 0.f;
   for (int k = 0; k < vc; ++k) {
     const int p = vb + k * vs;
     const ViewP vp = views[p];
+    // row shadow of every group, FP32 (DD3_GPU.cu:1446-1450)
+    const float dXf = Pxf - vp.sx, dYf = Pyf - vp.sy;
+    const float l2f = dXf * dXf + dYf * dYf;
+    const float rlS = rsqrtf(l2f) * c.S;
+    float vlo[ZG], vhi[ZG], tdif[ZG];
+    bool vok[ZG];
+    bool any = false;
+#pragma unroll
+    for (int g = 0; g < ZG; ++g) {
+      const float dZ = Pz[g] - vp.sz;
+      const float tU = fmaf(dZ + c.hz, rlS, c.cV);
+      const float tD = fmaf(dZ - c.hz, rlS, c.cV);
+      tdif[g] = tU - tD;
+      vlo[g] = fmaxf(fminf(tU, tD), -0.5f);
+      vhi[g] = fminf(fmaxf(tU, tD), DNVh);
+      vok[g] = live[g] && vlo[g] < vhi[g];
+      any |= (bool)__any(vok[g]);
+    }
+    if (!any) continue;
+    // fan-angle shadow, FP64 (DD3_GPU.cu:1447-1464); sy*dX - sx*dY == sy*Px - sx*Py
     const double2 s64 = src64[p];
     const double dX = Px - s64.x, dY = Py - s64.y;
     const double l2 = dX * dX + dY * dY;
     const double rl = rsqrt(l2);
-    const float rlf = (float)rl;
-    // detector-row shadow, per lane (DD3_GPU.cu:1449-1450)
-    const float dZ = Pz - vp.sz;
-    const float tU = (dZ + c.hz) * c.S * rlf + c.cV;
-    const float tD = (dZ - c.hz) * c.S * rlf + c.cV;
-    const float vlo = fmaxf(fminf(tU, tD), -0.5f);
-    const float vhi = fminf(fmaxf(tU, tD), DNVh);
-    const bool vok = vlo < vhi;
-    if (!__any(vok)) continue;
-    // fan-angle shadow (DD3_GPU.cu:1451-1464); sy*dX - sx*dY == sy*Px - sx*Py
     const double alpha = asin((s64.y * Px - s64.x * Py) * c64.inv_sid * rl);
     const double w = vp.wx ? dX : dY;
-    const double dA = w / l2 * c64.dx * 0.5;
+    const double dA = w * (rl * rl) * (c64.dx * 0.5);
     const double tL = (alpha - dA) * c64.ib + c64.cU;
     const double tR = (alpha + dA) * c64.ib + c64.cU;
     const double ulo = fmax(fmin(tL, tR), -0.5);
     const double uhi = fmin(fmax(tL, tR), DNUh);
-    if (!(ulo < uhi) || !vok) continue;
-    const float wf = (float)w;
-    const float cosVal = c.dx / wf * sqrtf((float)l2 + dZ * dZ);
-    const float sgn = ((tR - tL) * (double)(tU - tD) < 0.0) ? -1.f : 1.f;
+    if (!(ulo < uhi)) continue;
     const int u0 = (int)floor(ulo + 0.5);
     const int u1 = min((int)floor(uhi + 0.5), d.DNU - 1);
-    const int v0 = (int)floorf(vlo + 0.5f);
-    const int v1 = min((int)floorf(vhi + 0.5f), d.DNV - 1);
+    const float cw = c.dx / (float)w;
+    const bool lneg = (tR - tL) < 0.0;
     const float* pp = prj + (size_t)k * frame;
-    float s = 0.f;
-    for (int u = u0; u <= u1; ++u) {
-      const float wu = (float)(fmin(uhi, (double)u + 0.5) - fmax(ulo, (double)u - 0.5));
-      const float* row = pp + (size_t)u * d.DNV;
-      float g = 0.f;
-      for (int v = v0; v <= v1; ++v) g = fmaf(cell_w(vlo, vhi, (float)v), row[v], g);
-      s = fmaf(wu, g, s);
+#pragma unroll
+    for (int g = 0; g < ZG; ++g) {
+      if (!__any(vok[g])) continue;
+      if (vok[g]) {
+        const float dZ = Pz[g] - vp.sz;
+        const float cosVal = cw * sqrtf(l2f + dZ * dZ);
+        const float sgn = (lneg != (tdif[g] < 0.f)) ? -1.f : 1.f;
+        const int v0 = (int)floorf(vlo[g] + 0.5f);
+        const int v1 = min((int)floorf(vhi[g] + 0.5f), d.DNV - 1);
+        float s = 0.f;
+        for (int u = u0; u <= u1; ++u) {
+          const float wu = (float)(fmin(uhi, (double)u + 0.5) - fmax(ulo, (double)u - 0.5));
+          const float* row = pp + (size_t)u * d.DNV;
+          float gsum = 0.f;
+          for (int v = v0; v <= v1; ++v)
+            gsum = fmaf(cell_w(vlo[g], vhi[g], (float)v), row[v], gsum);
+          s = fmaf(wu, gsum, s);
+        }
+        acc[g] = fmaf(sgn * cosVal, s, acc[g]);
+      }
     }
-    acc = fmaf(sgn * cosVal, s, acc);
   }
-  if constexpr (UPDATE)
-    vol[o] = fmaf(acc, invCol[o], vol[o]);
-  else
-    vol[o] = acc;
+#pragma unroll
+  for (int g = 0; g < ZG; ++g) {
+    if (!live[g]) continue;
+    const int z = zb + 64 * g;
+    if constexpr (UPDATE)
+      vcol[z] = fmaf(acc[g], invCol[(size_t)col * d.ZN + z], vcol[z]);
+    else
+      vcol[z] = acc[g];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -221,7 +276,7 @@ __global__ __launch_bounds__(kBlock) void bp_ref_kernel(
 // its z cell among the row footprints; the overlap weights are those of the
 // forward projector.
 // ---------------------------------------------------------------------------
-template <bool UPDATE>
+template <bool UPDATE, int ZG>
 __global__ __launch_bounds__(kBlock) void bp_matched_kernel(
     const float* __restrict__ yp, float* __restrict__ vol, const ViewP* __restrict__ views,
     const float2* __restrict__ bnd, const float4* __restrict__ cols,
@@ -229,21 +284,30 @@ __global__ __launch_bounds__(kBlock) void bp_matched_kernel(
     int vb, int vs, int vc, const float* __restrict__ invCol) {
   const int lane = threadIdx.x & 63;
   const int col = blockIdx.x * kWaves + (threadIdx.x >> 6);
-  const int z = blockIdx.y * 64 + lane;
-  if (col >= d.XN * d.YN || z >= d.ZN) return;
+  if (col >= d.XN * d.YN) return;
+  const int zb = blockIdx.y * (64 * ZG) + lane;
   const int y = col / d.XN, x = col - y * d.XN;
-  const size_t o = (size_t)col * d.ZN + z;
+  float* vcol = vol + (size_t)col * d.ZN;
   const float m = (float)mask[col];
   if (m == 0.f) {
-    if constexpr (!UPDATE) vol[o] = 0.f;
+    if constexpr (!UPDATE) {
+#pragma unroll
+      for (int g = 0; g < ZG; ++g)
+        if (zb + 64 * g < d.ZN) vcol[zb + 64 * g] = 0.f;
+    }
     return;
   }
   const float Px = ((float)x - d.ocx) * d.dx;
   const float Py = ((float)y - d.ocy) * d.dx;
-  const float zlo = (float)z - 0.5f, zhi = (float)z + 0.5f;
+  float acc[ZG];
+  bool live[ZG];
+#pragma unroll
+  for (int g = 0; g < ZG; ++g) {
+    acc[g] = 0.f;
+    live[g] = zb + 64 * g < d.ZN;
+  }
   const size_t frame = (size_t)d.DNU * d.DNV;
   const int DNU = d.DNU, DNV = d.DNV;
-  float acc = 0.f;
   for (int k = 0; k < vc; ++k) {
     const int p = vb + k * vs;
     const ViewP vp = views[p];
@@ -255,7 +319,8 @@ __global__ __launch_bounds__(kBlock) void bp_matched_kernel(
     const float dir = (float)vp.bdir;
     const float cl = (vp.bdir > 0) ? (float)j - 0.5f : -((float)j + 0.5f);
     const float ch = cl + 1.0f;
-    // first guess from the fan angle of the voxel centre
+    // first guess from the fan angle of the voxel centre, then walk to the first
+    // column whose footprint (same FP32 expressions as fp_kernel) reaches cl
     const float dX = Px - vp.sx, dY = Py - vp.sy;
     const float beta = asinf((vp.sy * dX - vp.sx * dY) * c.inv_sid * rsqrtf(dX * dX + dY * dY));
     int u = (int)fminf(fmaxf(floorf((beta - d.beta0) / d.dbeta_g), 0.f), (float)(DNU - 1));
@@ -276,33 +341,42 @@ __global__ __launch_bounds__(kBlock) void bp_matched_kernel(
       if (wy > 0.f) {
         const float4 ct = ctab[u];
         const float r = fmaf(fi, ct.y, ct.x);
-        const float inv = 1.0f / ((su1 - su) * r);
-        // rows whose z footprint [az + Bz[v] r, az + Bz[v+1] r] meets [zlo, zhi]
-        int v = (int)fminf(fmaxf(floorf(((zlo - vp.az) / r - d.B0) / d.dB), 0.f),
-                           (float)(DNV - 1));
-        float zl = fmaf(r, Bz[v], vp.az);
-        while (v > 0 && zl > zlo) {
-          --v;
-          zl = fmaf(r, Bz[v], vp.az);
-        }
-        float zh = fmaf(r, Bz[v + 1], vp.az);
-        while (v < DNV - 1 && zh <= zlo) {
-          ++v;
-          zl = zh;
-          zh = fmaf(r, Bz[v + 1], vp.az);
-        }
+        const float wyi = wy * (1.0f / ((su1 - su) * r));
         const float* yu = yk + (size_t)u * DNV;
-        float g = 0.f;
-        for (; v < DNV && zl < zhi;) {
-          const float wz = fminf(zh, zhi) - fmaxf(zl, zlo);
-          if (wz > 0.f) g = fmaf(wz, yu[v], g);
-          ++v;
-          if (v < DNV) {
+        const float rinv = 1.0f / r;
+        const float zlo0 = fmaf(r, Bz[0], vp.az), zhiN = fmaf(r, Bz[DNV], vp.az);
+#pragma unroll
+        for (int g = 0; g < ZG; ++g) {
+          // rows whose z footprint [az + Bz[v] r, az + Bz[v+1] r] meets [zlo, zhi]
+          const float zlo = (float)(zb + 64 * g) - 0.5f, zhi = zlo + 1.0f;
+          const bool vis = live[g] && zhi > zlo0 && zlo < zhiN;
+          if (!__any(vis)) continue;
+          if (!vis) continue;
+          int v = (int)fminf(fmaxf(floorf(((zlo - vp.az) * rinv - d.B0) / d.dB), 0.f),
+                             (float)(DNV - 1));
+          float zl = fmaf(r, Bz[v], vp.az);
+          while (v > 0 && zl > zlo) {
+            --v;
+            zl = fmaf(r, Bz[v], vp.az);
+          }
+          float zh = fmaf(r, Bz[v + 1], vp.az);
+          while (v < DNV - 1 && zh <= zlo) {
+            ++v;
             zl = zh;
             zh = fmaf(r, Bz[v + 1], vp.az);
           }
+          float gsum = 0.f;
+          for (; v < DNV && zl < zhi;) {
+            const float wz = fminf(zh, zhi) - fmaxf(zl, zlo);
+            if (wz > 0.f) gsum = fmaf(wz, yu[v], gsum);
+            ++v;
+            if (v < DNV) {
+              zl = zh;
+              zh = fmaf(r, Bz[v + 1], vp.az);
+            }
+          }
+          acc[g] = fmaf(wyi, gsum, acc[g]);
         }
-        acc = fmaf(wy * inv, g, acc);
       }
       ++u;
       if (u < DNU) {
@@ -311,11 +385,16 @@ __global__ __launch_bounds__(kBlock) void bp_matched_kernel(
       }
     }
   }
-  acc *= m;
-  if constexpr (UPDATE)
-    vol[o] = fmaf(acc, invCol[o], vol[o]);
-  else
-    vol[o] = acc;
+#pragma unroll
+  for (int g = 0; g < ZG; ++g) {
+    if (!live[g]) continue;
+    const int z = zb + 64 * g;
+    const float a = acc[g] * m;
+    if constexpr (UPDATE)
+      vcol[z] = fmaf(a, invCol[(size_t)col * d.ZN + z], vcol[z]);
+    else
+      vcol[z] = a;
+  }
 }
 
 // yp[k][u][v] = y[k][u][v] * K(p,u,v) * invdB[v]: the per-ray factors of fp_kernel.
@@ -423,33 +502,61 @@ int launch_fp(const Plan& P, const float* vol, float* out, int vb, int vs, int v
   dim3 grid((P.d.DNU * P.d.DNV + kBlock - 1) / kBlock, vc);
   if (b) {
     hipLaunchKernelGGL(fp_kernel<true>, grid, dim3(kBlock), 0, st, vol, out, P.views, P.bnd,
-                       P.cols, P.Bz, P.h2, P.invdB, P.mask, P.d, vb, vs, b, invRow, partials);
+                       P.cols, P.Bz, P.h2, P.invdB, P.mask, P.mext, P.d, vb, vs, b, invRow,
+                       partials);
   } else {
     hipLaunchKernelGGL(fp_kernel<false>, grid, dim3(kBlock), 0, st, vol, out, P.views, P.bnd,
-                       P.cols, P.Bz, P.h2, P.invdB, P.mask, P.d, vb, vs, nullptr, nullptr,
-                       nullptr);
+                       P.cols, P.Bz, P.h2, P.invdB, P.mask, P.mext, P.d, vb, vs, nullptr,
+                       nullptr, nullptr);
   }
   return check_launch("fp_kernel");
 }
 
 int launch_bp(const Plan& P, const float* prj, float* vol, int vb, int vs, int vc, int kind,
               hipStream_t st, const float* invCol, int accumulate) {
-  dim3 grid((P.d.XN * P.d.YN + kWaves - 1) / kWaves, (P.d.ZN + 63) / 64);
+  const int zg = std::min(4, (P.d.ZN + 63) / 64);
+  dim3 rgrid((P.d.XN * P.d.YN + kWaves - 1) / kWaves, (P.d.ZN + 64 * zg - 1) / (64 * zg));
   if (kind == CTR_BP_MATCHED) {
-    if (accumulate)
-      hipLaunchKernelGGL(bp_matched_kernel<true>, grid, dim3(kBlock), 0, st, prj, vol, P.views,
-                         P.bnd, P.cols, P.Bz, P.mask, P.d, P.bc, vb, vs, vc, invCol);
-    else
-      hipLaunchKernelGGL(bp_matched_kernel<false>, grid, dim3(kBlock), 0, st, prj, vol, P.views,
-                         P.bnd, P.cols, P.Bz, P.mask, P.d, P.bc, vb, vs, vc, nullptr);
+#define CTR_BPM(UPD, ZGV, IC)                                                                  \
+  hipLaunchKernelGGL((bp_matched_kernel<UPD, ZGV>), rgrid, dim3(kBlock), 0, st, prj, vol, P.views, \
+                     P.bnd, P.cols, P.Bz, P.mask, P.d, P.bc, vb, vs, vc, IC)
+    if (accumulate) {
+      switch (zg) {
+        case 1: CTR_BPM(true, 1, invCol); break;
+        case 2: CTR_BPM(true, 2, invCol); break;
+        case 3: CTR_BPM(true, 3, invCol); break;
+        default: CTR_BPM(true, 4, invCol); break;
+      }
+    } else {
+      switch (zg) {
+        case 1: CTR_BPM(false, 1, nullptr); break;
+        case 2: CTR_BPM(false, 2, nullptr); break;
+        case 3: CTR_BPM(false, 3, nullptr); break;
+        default: CTR_BPM(false, 4, nullptr); break;
+      }
+    }
+#undef CTR_BPM
     return check_launch("bp_matched_kernel");
   }
-  if (accumulate)
-    hipLaunchKernelGGL(bp_ref_kernel<true>, grid, dim3(kBlock), 0, st, prj, vol, P.views, P.src64,
-                       P.mask, P.d, P.bc, P.bc64, vb, vs, vc, invCol);
-  else
-    hipLaunchKernelGGL(bp_ref_kernel<false>, grid, dim3(kBlock), 0, st, prj, vol, P.views,
-                       P.src64, P.mask, P.d, P.bc, P.bc64, vb, vs, vc, nullptr);
+#define CTR_BPREF(UPD, ZGV, IC)                                                             \
+  hipLaunchKernelGGL((bp_ref_kernel<UPD, ZGV>), rgrid, dim3(kBlock), 0, st, prj, vol, P.views, \
+                     P.src64, P.mask, P.d, P.bc, P.bc64, vb, vs, vc, IC)
+  if (accumulate) {
+    switch (zg) {
+      case 1: CTR_BPREF(true, 1, invCol); break;
+      case 2: CTR_BPREF(true, 2, invCol); break;
+      case 3: CTR_BPREF(true, 3, invCol); break;
+      default: CTR_BPREF(true, 4, invCol); break;
+    }
+  } else {
+    switch (zg) {
+      case 1: CTR_BPREF(false, 1, nullptr); break;
+      case 2: CTR_BPREF(false, 2, nullptr); break;
+      case 3: CTR_BPREF(false, 3, nullptr); break;
+      default: CTR_BPREF(false, 4, nullptr); break;
+    }
+  }
+#undef CTR_BPREF
   return check_launch("bp_ref_kernel");
 }
 

@@ -225,7 +225,23 @@ static int build_tables(const ctr_geometry* g, Plan& P) {
   std::vector<uint8_t> mask((size_t)g->XN * g->YN, 1);
   if (g->mask) std::memcpy(mask.data(), g->mask, mask.size());
 
+  // per-slice extent of the non-zero mask: x-slices (cells along y), then y-slices
+  std::vector<float2> mext(g->XN + g->YN);
+  for (int x = 0; x < g->XN; ++x) {
+    int lo = g->YN, hi = -1;
+    for (int y = 0; y < g->YN; ++y)
+      if (mask[(size_t)y * g->XN + x]) { lo = std::min(lo, y); hi = std::max(hi, y); }
+    mext[x] = (hi >= lo) ? make_float2(lo - 0.5f, hi + 0.5f) : make_float2(0.f, -1.f);
+  }
+  for (int y = 0; y < g->YN; ++y) {
+    int lo = g->XN, hi = -1;
+    for (int x = 0; x < g->XN; ++x)
+      if (mask[(size_t)y * g->XN + x]) { lo = std::min(lo, x); hi = std::max(hi, x); }
+    mext[g->XN + y] = (hi >= lo) ? make_float2(lo - 0.5f, hi + 0.5f) : make_float2(0.f, -1.f);
+  }
+
   int rc;
+  if ((rc = upload(&P.mext, mext))) return rc;
   if ((rc = upload(&P.views, views))) return rc;
   if ((rc = upload(&P.src64, src64))) return rc;
   if ((rc = upload(&P.bnd, bnd))) return rc;
@@ -247,6 +263,7 @@ static void free_plan(Plan* P) {
   (void)hipFree(P->h2);
   (void)hipFree(P->invdB);
   (void)hipFree(P->mask);
+  (void)hipFree(P->mext);
   (void)hipFree(P->scratch_prj);
   (void)hipFree(P->partials);
   delete P;
