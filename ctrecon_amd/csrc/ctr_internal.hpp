@@ -73,6 +73,7 @@ struct Plan {
   float* h2 = nullptr;         // [DNV]        (zds_v - z0)^2
   float* invdB = nullptr;      // [DNV]        1/(Bz[v+1]-Bz[v])
   uint8_t* mask = nullptr;     // [XN*YN]
+  float* maskf = nullptr;      // [XN*YN]      mask byte as float (the forward multiplier)
   float2* mext = nullptr;      // [XN + YN]    clip interval of the mask per x- then y-slice
   // host copies (for culling and solvers)
   std::vector<ViewP> h_views;

@@ -27,6 +27,27 @@ __device__ __forceinline__ float cell_w(float lo, float hi, float c) {
   return fminf(hi, c + 0.5f) - fmaxf(lo, c - 0.5f);
 }
 
+// Raw buffer view of a float array: 32-bit byte offsets, and reads past
+// num_records return 0 instead of touching memory outside the array.
+struct BufF32 {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ BufF32(const float* p, uint32_t n_floats)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, (int)(n_floats * 4u),
+                                            0x00020000)) {}
+  __device__ __forceinline__ float operator[](int i) const {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, i * 4, 0, 0));
+  }
+};
+
+// Overlap weights of [lo, hi] (lo < hi, clipped to the array) with the cells
+// c0 = floor(lo + .5) ... c0 + n - 1: first, interior (1) and last.
+__device__ __forceinline__ void edge_weights(float lo, float hi, int c0, int n, float& wfirst,
+                                             float& wlast) {
+  const float e0 = (float)c0 + 0.5f;  // upper edge of the first cell
+  wfirst = (n == 1) ? hi - lo : e0 - lo;
+  wlast = hi - ((float)(c0 + n - 1) - 0.5f);
+}
+
 // Block-wide sum of two doubles into partials[2*block + {0,1}].
 __device__ __forceinline__ void block_sum2(double a, double b, double* partials, int block) {
   __shared__ double red[2][kWaves];
@@ -74,7 +95,7 @@ __global__ __launch_bounds__(kBlock) void fp_kernel(
     const float* __restrict__ vol, float* __restrict__ out, const ViewP* __restrict__ views,
     const float2* __restrict__ bnd, const float4* __restrict__ cols,
     const float* __restrict__ Bz, const float* __restrict__ h2,
-    const float* __restrict__ invdB, const uint8_t* __restrict__ mask,
+    const float* __restrict__ invdB, const float* __restrict__ maskf,
     const float2* __restrict__ mext, Dims d, int vb, int vs,
     const float* __restrict__ b, const float* __restrict__ invRow,
     double* __restrict__ partials) {
@@ -100,41 +121,67 @@ __global__ __launch_bounds__(kBlock) void fp_kernel(
       NS = d.YN; NI = d.XN; sS = d.XN * d.ZN; cS = d.ZN; mS = d.XN; mC = 1;
     }
     const float ZNh = (float)d.ZN - 0.5f;
+    const int ZN1 = d.ZN - 1;
     const float2* me = mext + (vp.xmarch ? 0 : d.XN);
+    const BufF32 V(vol, (uint32_t)(d.XN * d.YN * d.ZN));
     float acc = 0.f;
     for (int ii = 0; ii < NS; ++ii) {
       const float fi = (float)ii;
       const float tl = fmaf(fi, bl.y, bl.x);
       const float tr = fmaf(fi, br.y, br.x);
-      const float r = fmaf(fi, ct.y, ct.x);
       float ylo = fminf(tl, tr), yhi = fmaxf(tl, tr);
-      const float inv = 1.0f / ((yhi - ylo) * r);
       // clip to the mask's extent on this slice (inside [-0.5, NI - 0.5]); cells
       // outside it are zero
       const float2 e = me[ii];
-      ylo = fmaxf(ylo, e.x);
-      yhi = fminf(yhi, e.y);
-      if (!(ylo < yhi)) continue;
+      const float clo = fmaxf(ylo, e.x), chi = fminf(yhi, e.y);
+      if (!(clo < chi)) continue;
+      const float r = fmaf(fi, ct.y, ct.x);
+      const float inv = __builtin_amdgcn_rcpf((yhi - ylo) * r);
       float zl = fmaf(r, Bv, vp.az), zh = fmaf(r, Bv1, vp.az);
       zl = fmaxf(zl, -0.5f);
       zh = fminf(zh, ZNh);
-      if (!(zl < zh)) continue;
-      const int k0 = (int)floorf(zl + 0.5f);
-      const int k1 = min((int)floorf(zh + 0.5f), d.ZN - 1);
-      const int j0 = (int)floorf(ylo + 0.5f);
-      const int j1 = min((int)floorf(yhi + 0.5f), NI - 1);
+      const bool zok = zl < zh;
+      if (!__any(zok)) continue;
+      const int k0 = min((int)floorf(zl + 0.5f), ZN1);
+      const int nk = zok ? min((int)floorf(zh + 0.5f), ZN1) - k0 + 1 : 0;
+      float wz0, wzl;
+      edge_weights(zl, zh, k0, nk, wz0, wzl);
+      const float wz1 = (nk == 2) ? wzl : (nk >= 3 ? 1.f : 0.f);
+      const float wz2 = (nk == 3) ? wzl : 0.f;
+      if (nk == 0) wz0 = 0.f;
+      const int j0 = (int)floorf(clo + 0.5f);
+      const int nj = min((int)floorf(chi + 0.5f), NI - 1) - j0 + 1;
+      float wj0, wjl;
+      edge_weights(clo, chi, j0, nj, wj0, wjl);
+      int o = ii * sS + j0 * cS + k0;
+      int mo = ii * mS + j0 * mC;
+      // unused taps re-read the first cell (weight 0), never a neighbouring column
+      const int t1 = nk > 1 ? 1 : 0, t2 = nk > 2 ? 2 : 0;
       float s = 0.f;
-      for (int j = j0; j <= j1; ++j) {
-        const float m = (float)mask[ii * mS + j * mC];
-        const float wy = cell_w(ylo, yhi, (float)j) * m;
-        const float* colp = vol + (size_t)ii * sS + (size_t)j * cS;
-        float g = 0.f;
-        for (int kk = k0; kk <= k1; ++kk) g = fmaf(cell_w(zl, zh, (float)kk), colp[kk], g);
-        s = fmaf(wy, g, s);
+      if (!__any(nk > 3)) {
+        for (int ij = 0; ij < nj; ++ij) {
+          const float wy = ((ij == 0) ? wj0 : (ij == nj - 1 ? wjl : 1.f)) * maskf[mo];
+          float g = wz0 * V[o];
+          g = fmaf(wz1, V[o + t1], g);
+          g = fmaf(wz2, V[o + t2], g);
+          s = fmaf(wy, g, s);
+          o += cS;
+          mo += mC;
+        }
+      } else {
+        for (int ij = 0; ij < nj; ++ij) {
+          const float wy = ((ij == 0) ? wj0 : (ij == nj - 1 ? wjl : 1.f)) * maskf[mo];
+          float g = 0.f;
+          for (int ik = 0; ik < nk; ++ik)
+            g = fmaf((ik == 0) ? wz0 : (ik == nk - 1 ? wzl : 1.f), V[o + ik], g);
+          s = fmaf(wy, g, s);
+          o += cS;
+          mo += mC;
+        }
       }
       acc = fmaf(s, inv, acc);
     }
-    const float K = ct.w * sqrtf(ct.z + h2[v]) * invdB[v];
+    const float K = ct.w * __builtin_amdgcn_sqrtf(ct.z + h2[v]) * invdB[v];
     const float val = acc * K;
     const size_t o = ((size_t)k * d.DNU + u) * d.DNV + v;
     if constexpr (FUSED) {
@@ -194,23 +241,24 @@ __global__ __launch_bounds__(kBlock) void bp_ref_kernel(
   }
   const double DNUh = (double)d.DNU - 0.5;
   const float DNVh = (float)d.DNV - 0.5f;
-  const size_t frame = (size_t)d.DNU * d.DNV;
+  const int DNV = d.DNV;
+  const uint32_t frame = (uint32_t)(d.DNU * d.DNV);
   for (int k = 0; k < vc; ++k) {
     const int p = vb + k * vs;
     const ViewP vp = views[p];
     // row shadow of every group, FP32 (DD3_GPU.cu:1446-1450)
     const float dXf = Pxf - vp.sx, dYf = Pyf - vp.sy;
     const float l2f = dXf * dXf + dYf * dYf;
-    const float rlS = rsqrtf(l2f) * c.S;
-    float vlo[ZG], vhi[ZG], tdif[ZG];
-    bool vok[ZG];
+    const float rlS = __builtin_amdgcn_rsqf(l2f) * c.S;
+    float vlo[ZG], vhi[ZG];
+    bool vok[ZG], vneg[ZG];
     bool any = false;
 #pragma unroll
     for (int g = 0; g < ZG; ++g) {
       const float dZ = Pz[g] - vp.sz;
       const float tU = fmaf(dZ + c.hz, rlS, c.cV);
       const float tD = fmaf(dZ - c.hz, rlS, c.cV);
-      tdif[g] = tU - tD;
+      vneg[g] = tU < tD;
       vlo[g] = fmaxf(fminf(tU, tD), -0.5f);
       vhi[g] = fminf(fmaxf(tU, tD), DNVh);
       vok[g] = live[g] && vlo[g] < vhi[g];
@@ -231,30 +279,55 @@ __global__ __launch_bounds__(kBlock) void bp_ref_kernel(
     const double uhi = fmin(fmax(tL, tR), DNUh);
     if (!(ulo < uhi)) continue;
     const int u0 = (int)floor(ulo + 0.5);
-    const int u1 = min((int)floor(uhi + 0.5), d.DNU - 1);
-    const float cw = c.dx / (float)w;
-    const bool lneg = (tR - tL) < 0.0;
-    const float* pp = prj + (size_t)k * frame;
+    const int nu = min((int)floor(uhi + 0.5), d.DNU - 1) - u0 + 1;
+    // column weights: first / last partial, interior 1 (fractions taken in FP64)
+    const float fl = (float)(((double)u0 + 0.5) - ulo);
+    const float fh = (float)(uhi - ((double)(u0 + nu - 1) - 0.5));
+    const float wu0 = (nu == 1) ? (float)(uhi - ulo) : fl;
+    const float cw = c.dx * __builtin_amdgcn_rcpf((float)w);
+    const bool lneg = tR < tL;
+    const BufF32 pp(prj + (size_t)k * frame, frame);
 #pragma unroll
     for (int g = 0; g < ZG; ++g) {
       if (!__any(vok[g])) continue;
-      if (vok[g]) {
-        const float dZ = Pz[g] - vp.sz;
-        const float cosVal = cw * sqrtf(l2f + dZ * dZ);
-        const float sgn = (lneg != (tdif[g] < 0.f)) ? -1.f : 1.f;
-        const int v0 = (int)floorf(vlo[g] + 0.5f);
-        const int v1 = min((int)floorf(vhi[g] + 0.5f), d.DNV - 1);
-        float s = 0.f;
-        for (int u = u0; u <= u1; ++u) {
-          const float wu = (float)(fmin(uhi, (double)u + 0.5) - fmax(ulo, (double)u - 0.5));
-          const float* row = pp + (size_t)u * d.DNV;
+      if (!vok[g]) continue;
+      const float dZ = Pz[g] - vp.sz;
+      const float cosVal = cw * __builtin_amdgcn_sqrtf(fmaf(dZ, dZ, l2f));
+      const float sgn = (lneg != vneg[g]) ? -1.f : 1.f;
+      const int v0 = (int)floorf(vlo[g] + 0.5f);
+      const int nv = min((int)floorf(vhi[g] + 0.5f), DNV - 1) - v0 + 1;
+      float wv0, wvl;
+      edge_weights(vlo[g], vhi[g], v0, nv, wv0, wvl);
+      float s = 0.f;
+      if (nu <= 4 && nv <= 3) {
+        const float wv1 = (nv == 2) ? wvl : (nv == 3 ? 1.f : 0.f);
+        const float wv2 = (nv == 3) ? wvl : 0.f;
+        const int t1 = nv > 1 ? 1 : 0, t2 = nv > 2 ? 2 : 0;
+        int o = u0 * DNV + v0;
+#pragma unroll
+        for (int iu = 0; iu < 4; ++iu) {
+          if (iu < nu) {
+            const float wu = (iu == 0) ? wu0 : (iu == nu - 1 ? fh : 1.f);
+            float gsum = wv0 * pp[o];
+            gsum = fmaf(wv1, pp[o + t1], gsum);
+            gsum = fmaf(wv2, pp[o + t2], gsum);
+            s = fmaf(wu, gsum, s);
+            o += DNV;
+          }
+        }
+      } else {
+        for (int iu = 0; iu < nu; ++iu) {
+          const float wu = (iu == 0) ? wu0 : (iu == nu - 1 ? fh : 1.f);
+          const int o = (u0 + iu) * DNV;
           float gsum = 0.f;
-          for (int v = v0; v <= v1; ++v)
-            gsum = fmaf(cell_w(vlo[g], vhi[g], (float)v), row[v], gsum);
+          for (int iv = 0; iv < nv; ++iv) {
+            const float wv = (iv == 0) ? wv0 : (iv == nv - 1 ? wvl : 1.f);
+            gsum = fmaf(wv, pp[o + v0 + iv], gsum);
+          }
           s = fmaf(wu, gsum, s);
         }
-        acc[g] = fmaf(sgn * cosVal, s, acc[g]);
       }
+      acc[g] = fmaf(sgn * cosVal, s, acc[g]);
     }
   }
 #pragma unroll
@@ -341,7 +414,7 @@ __global__ __launch_bounds__(kBlock) void bp_matched_kernel(
       if (wy > 0.f) {
         const float4 ct = ctab[u];
         const float r = fmaf(fi, ct.y, ct.x);
-        const float wyi = wy * (1.0f / ((su1 - su) * r));
+        const float wyi = wy * __builtin_amdgcn_rcpf((su1 - su) * r);
         const float* yu = yk + (size_t)u * DNV;
         const float rinv = 1.0f / r;
         const float zlo0 = fmaf(r, Bz[0], vp.az), zhiN = fmaf(r, Bz[DNV], vp.az);
@@ -502,11 +575,11 @@ int launch_fp(const Plan& P, const float* vol, float* out, int vb, int vs, int v
   dim3 grid((P.d.DNU * P.d.DNV + kBlock - 1) / kBlock, vc);
   if (b) {
     hipLaunchKernelGGL(fp_kernel<true>, grid, dim3(kBlock), 0, st, vol, out, P.views, P.bnd,
-                       P.cols, P.Bz, P.h2, P.invdB, P.mask, P.mext, P.d, vb, vs, b, invRow,
+                       P.cols, P.Bz, P.h2, P.invdB, P.maskf, P.mext, P.d, vb, vs, b, invRow,
                        partials);
   } else {
     hipLaunchKernelGGL(fp_kernel<false>, grid, dim3(kBlock), 0, st, vol, out, P.views, P.bnd,
-                       P.cols, P.Bz, P.h2, P.invdB, P.mask, P.mext, P.d, vb, vs, nullptr,
+                       P.cols, P.Bz, P.h2, P.invdB, P.maskf, P.mext, P.d, vb, vs, nullptr,
                        nullptr, nullptr);
   }
   return check_launch("fp_kernel");

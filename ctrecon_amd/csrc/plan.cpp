@@ -240,8 +240,11 @@ static int build_tables(const ctr_geometry* g, Plan& P) {
     mext[g->XN + y] = (hi >= lo) ? make_float2(lo - 0.5f, hi + 0.5f) : make_float2(0.f, -1.f);
   }
 
+  std::vector<float> maskf(mask.begin(), mask.end());
+
   int rc;
   if ((rc = upload(&P.mext, mext))) return rc;
+  if ((rc = upload(&P.maskf, maskf))) return rc;
   if ((rc = upload(&P.views, views))) return rc;
   if ((rc = upload(&P.src64, src64))) return rc;
   if ((rc = upload(&P.bnd, bnd))) return rc;
@@ -264,6 +267,7 @@ static void free_plan(Plan* P) {
   (void)hipFree(P->invdB);
   (void)hipFree(P->mask);
   (void)hipFree(P->mext);
+  (void)hipFree(P->maskf);
   (void)hipFree(P->scratch_prj);
   (void)hipFree(P->partials);
   delete P;
