@@ -160,7 +160,7 @@ def main():
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get(dominant + "<true>")
+        traffic = json.load(open(tpath)).get(dominant)
     out = {
         "metric": METRIC,
         "value": t_target,
