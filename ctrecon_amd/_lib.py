@@ -20,13 +20,16 @@ CTR_ERR_NOMEM = 4
 CTR_ERR_NUMERIC = 5
 CTR_BP_REFERENCE = 0
 CTR_BP_MATCHED = 1
+CTR_METHOD_DIRECT = 0
+CTR_METHOD_SAT = 1
 
 _NAMES = {0: "CTR_OK", 1: "CTR_ERR_ARG", 2: "CTR_ERR_HIP", 3: "CTR_ERR_UNSUPPORTED",
           4: "CTR_ERR_NOMEM", 5: "CTR_ERR_NUMERIC"}
 
 # Every symbol include/ctrecon.h declares.
 EXPORTS = [
-    "ctr_dd3_proj", "ctr_dd3_back", "ctr_plan_create", "ctr_plan_destroy", "ctr_plan_fp",
+    "ctr_dd3_proj", "ctr_dd3_back", "ctr_plan_create", "ctr_plan_destroy", "ctr_plan_set_method",
+    "ctr_plan_fp",
     "ctr_plan_bp", "ctr_os_sart", "ctr_os_sart_run", "ctr_cone_voxels", "ctr_conjgrad",
     "ctr_residual", "ctr_device_count",
     "ctr_malloc", "ctr_free", "ctr_memcpy_h2d", "ctr_memcpy_d2h", "ctr_memset",
@@ -77,6 +80,7 @@ def lib():
         L.ctr_dd3_back.argtypes = geo + [p, p, f, f, p, i, i, i]
         L.ctr_plan_create.argtypes = [ctypes.POINTER(CtrGeometry), i, ctypes.POINTER(p)]
         L.ctr_plan_destroy.argtypes = [p]
+        L.ctr_plan_set_method.argtypes = [p, i]
         L.ctr_plan_fp.argtypes = [p, p, p, i, i, i, p]
         L.ctr_plan_bp.argtypes = [p, p, p, i, i, i, i, p]
         L.ctr_os_sart.argtypes = [p, p, p, i, i, i, p]

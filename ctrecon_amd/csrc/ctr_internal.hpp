@@ -78,7 +78,13 @@ struct Plan {
   // host copies (for culling and solvers)
   std::vector<ViewP> h_views;
   HostGeo hg;
+  // projector method: CTR_METHOD_DIRECT or CTR_METHOD_SAT
+  int method = CTR_METHOD_DIRECT;
   // scratch
+  float* sat_vol = nullptr;         // volume integral images (hi then lo planes)
+  size_t sat_vol_n = 0;
+  float* sat_sino = nullptr;        // sinogram integral images of one launch
+  size_t sat_sino_n = 0;
   float* scratch_prj = nullptr;     // [max_views][DNU][DNV]
   size_t scratch_prj_views = 0;
   double* partials = nullptr;       // reduction partials
@@ -113,5 +119,13 @@ int launch_dot(const float* a, const float* b, size_t n, double* partials, int n
                double* result, hipStream_t st);
 int launch_reduce_pairs(const double* partials, int nparts, double* result, hipStream_t st);
 int fp_partial_blocks(const Plan& P, int vc);
+// integral-image method (sat.hip)
+size_t sat_volume_floats(const Dims& d);
+size_t sat_sino_floats(const Dims& d, int vc);
+int launch_volume_sat(const Plan& P, const float* vol, float* sat, hipStream_t st);
+int launch_fp_sat(const Plan& P, const float* sat, float* out, int vb, int vs, int vc,
+                  hipStream_t st, const float* b, const float* invRow, double* partials);
+int launch_bp_sat(const Plan& P, const float* prj, float* sat, float* vol, int vb, int vs,
+                  int vc, hipStream_t st, const float* invCol, int accumulate);
 
 }  // namespace ctr

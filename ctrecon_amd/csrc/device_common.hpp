@@ -1,0 +1,70 @@
+// Device helpers shared by the HIP translation units of the library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ctr {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+
+// Overlap of [lo, hi] with the cell [c - 0.5, c + 0.5].
+__device__ __forceinline__ float cell_w(float lo, float hi, float c) {
+  return fminf(hi, c + 0.5f) - fmaxf(lo, c - 0.5f);
+}
+
+// Raw buffer view of a float array: 32-bit byte offsets, and reads past
+// num_records return 0 instead of touching memory outside the array.
+struct BufF32 {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ BufF32(const float* p, uint32_t n_floats)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, (int)(n_floats * 4u),
+                                            0x00020000)) {}
+  __device__ __forceinline__ float operator[](int i) const {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, i * 4, 0, 0));
+  }
+};
+
+// Overlap weights of [lo, hi] (lo < hi, clipped to the array) with the cells
+// c0 = floor(lo + .5) ... c0 + n - 1: first, interior (1) and last.
+__device__ __forceinline__ void edge_weights(float lo, float hi, int c0, int n, float& wfirst,
+                                             float& wlast) {
+  const float e0 = (float)c0 + 0.5f;  // upper edge of the first cell
+  wfirst = (n == 1) ? hi - lo : e0 - lo;
+  wlast = hi - ((float)(c0 + n - 1) - 0.5f);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2):
+// remap the flat workgroup id so that every XCD works through one contiguous range
+// of logical blocks (neighbouring detector columns of the same view).
+__device__ __forceinline__ int xcd_logical_block(int hw, int G) {
+  const int q = G >> 3, r = G & 7, xcd = hw & 7, loc = hw >> 3;
+  return (xcd < r) ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
+}
+
+// Double-float (hi + lo) value: an FP32 pair carrying about 48 bits.
+struct DF {
+  float hi, lo;
+};
+
+// Error-free sum of two floats (Knuth's TwoSum): s + e == a + b exactly.
+__device__ __forceinline__ void two_sum(float a, float b, float& s, float& e) {
+  s = a + b;
+  const float bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+// Compensated addition of two double-float values.
+__device__ __forceinline__ DF df_add(DF a, DF b) {
+  float s, e;
+  two_sum(a.hi, b.hi, s, e);
+  e += a.lo + b.lo;
+  DF r;
+  r.hi = s + e;
+  r.lo = e - (r.hi - s);
+  return r;
+}
+
+}  // namespace ctr

@@ -15,38 +15,10 @@
 #include <algorithm>
 
 #include "ctr_internal.hpp"
+#include "device_common.hpp"
 
 namespace ctr {
 namespace {
-
-constexpr int kBlock = 256;
-constexpr int kWaves = kBlock / 64;
-
-// Overlap of [lo, hi] with the cell [c - 0.5, c + 0.5].
-__device__ __forceinline__ float cell_w(float lo, float hi, float c) {
-  return fminf(hi, c + 0.5f) - fmaxf(lo, c - 0.5f);
-}
-
-// Raw buffer view of a float array: 32-bit byte offsets, and reads past
-// num_records return 0 instead of touching memory outside the array.
-struct BufF32 {
-  __amdgpu_buffer_rsrc_t r;
-  __device__ __forceinline__ BufF32(const float* p, uint32_t n_floats)
-      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, (int)(n_floats * 4u),
-                                            0x00020000)) {}
-  __device__ __forceinline__ float operator[](int i) const {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, i * 4, 0, 0));
-  }
-};
-
-// Overlap weights of [lo, hi] (lo < hi, clipped to the array) with the cells
-// c0 = floor(lo + .5) ... c0 + n - 1: first, interior (1) and last.
-__device__ __forceinline__ void edge_weights(float lo, float hi, int c0, int n, float& wfirst,
-                                             float& wlast) {
-  const float e0 = (float)c0 + 0.5f;  // upper edge of the first cell
-  wfirst = (n == 1) ? hi - lo : e0 - lo;
-  wlast = hi - ((float)(c0 + n - 1) - 0.5f);
-}
 
 // Block-wide sum of two doubles into partials[2*block + {0,1}].
 __device__ __forceinline__ void block_sum2(double a, double b, double* partials, int block) {
@@ -82,14 +54,6 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* partials,
 // invRow by the launch's compact view index, NULL for 1) and per-block sums of
 // (b - A x)^2 and b^2.
 // ---------------------------------------------------------------------------
-// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2):
-// remap the flat workgroup id so that every XCD works through one contiguous range
-// of logical blocks (neighbouring detector columns of the same view).
-__device__ __forceinline__ int xcd_logical_block(int hw, int G) {
-  const int q = G >> 3, r = G & 7, xcd = hw & 7, loc = hw >> 3;
-  return (xcd < r) ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
-}
-
 template <bool FUSED>
 __global__ __launch_bounds__(kBlock) void fp_kernel(
     const float* __restrict__ vol, float* __restrict__ out, const ViewP* __restrict__ views,

@@ -269,6 +269,8 @@ static void free_plan(Plan* P) {
   (void)hipFree(P->mext);
   (void)hipFree(P->maskf);
   (void)hipFree(P->scratch_prj);
+  (void)hipFree(P->sat_vol);
+  (void)hipFree(P->sat_sino);
   (void)hipFree(P->partials);
   delete P;
 }
@@ -303,8 +305,38 @@ static int check_views(const Plan& P, int vb, int vs, int vc) {
   return CTR_OK;
 }
 
+static int ensure_buf(float** p, size_t* have, size_t need, const char* what) {
+  if (*have >= need) return CTR_OK;
+  (void)hipFree(*p);
+  *p = nullptr;
+  *have = 0;
+  if (hipMalloc((void**)p, need * sizeof(float)) != hipSuccess)
+    return fail(CTR_ERR_NOMEM, std::string("cannot allocate ") + what);
+  *have = need;
+  return CTR_OK;
+}
+
+// Forward projection with the plan's method (fused residual when b != NULL).
+static int fp_any(Plan& P, const float* vol, float* out, int vb, int vs, int vc, hipStream_t st,
+                  const float* b, const float* invRow, double* partials) {
+  if (P.method == CTR_METHOD_SAT) {
+    int rc = ensure_buf(&P.sat_vol, &P.sat_vol_n, 2 * sat_volume_floats(P.d),
+                        "volume integral images");
+    if (rc) return rc;
+    if ((rc = launch_volume_sat(P, vol, P.sat_vol, st))) return rc;
+    return launch_fp_sat(P, P.sat_vol, out, vb, vs, vc, st, b, invRow, partials);
+  }
+  return launch_fp(P, vol, out, vb, vs, vc, st, b, invRow, partials);
+}
+
 static int bp_any(Plan& P, const float* prj, float* vol, int vb, int vs, int vc, int kind,
                   hipStream_t st, const float* invCol, int accumulate) {
+  if (kind == CTR_BP_REFERENCE && P.method == CTR_METHOD_SAT) {
+    int rc = ensure_buf(&P.sat_sino, &P.sat_sino_n, 2 * sat_sino_floats(P.d, vc),
+                        "sinogram integral images");
+    if (rc) return rc;
+    return launch_bp_sat(P, prj, P.sat_sino, vol, vb, vs, vc, st, invCol, accumulate);
+  }
   if (kind == CTR_BP_MATCHED) {
     int rc = ensure_scratch_prj(P, vc);
     if (rc) return rc;
@@ -430,6 +462,15 @@ int ctr_plan_destroy(ctr_plan* plan) {
   return CTR_OK;
 }
 
+int ctr_plan_set_method(ctr_plan* plan, int method) {
+  Plan* P = get(plan);
+  if (!P) return fail(CTR_ERR_ARG, "NULL plan");
+  if (method != CTR_METHOD_DIRECT && method != CTR_METHOD_SAT)
+    return fail(CTR_ERR_ARG, "method must be CTR_METHOD_DIRECT or CTR_METHOD_SAT");
+  P->method = method;
+  return CTR_OK;
+}
+
 int ctr_plan_fp(ctr_plan* plan, const float* d_vol, float* d_prj, int view_begin,
                 int view_stride, int view_count, void* stream) {
   Plan* P = get(plan);
@@ -437,7 +478,7 @@ int ctr_plan_fp(ctr_plan* plan, const float* d_vol, float* d_prj, int view_begin
   int rc = check_views(*P, view_begin, view_stride, view_count);
   if (rc) return rc;
   CTR_HIP(hipSetDevice(P->device));
-  return launch_fp(*P, d_vol, d_prj, view_begin, view_stride, view_count, (hipStream_t)stream,
+  return fp_any(*P, d_vol, d_prj, view_begin, view_stride, view_count, (hipStream_t)stream,
                    nullptr, nullptr, nullptr);
 }
 
@@ -470,7 +511,7 @@ int ctr_residual(ctr_plan* plan, const float* d_x, const float* d_prj, double* o
   double rr = 0, bb = 0;
   for (int v0 = 0; v0 < PN; v0 += chunk) {
     const int vc = std::min(chunk, PN - v0);
-    if ((rc = launch_fp(*P, d_x, P->scratch_prj, v0, 1, vc, nullptr, d_prj, nullptr,
+    if ((rc = fp_any(*P, d_x, P->scratch_prj, v0, 1, vc, nullptr, d_prj, nullptr,
                         P->partials)))
       return rc;
     if ((rc = launch_reduce_pairs(P->partials, fp_partial_blocks(*P, vc), sums.d(), nullptr)))
@@ -526,7 +567,7 @@ int ctr_os_sart_run(ctr_plan* plan, float* d_x, const float* d_prj, int osNum, i
     const int vc = (d.PN - s + osNum - 1) / osNum;
     rowOff[s] = off;
     float* rI = rowInv.f() + off * frame;
-    if ((rc = launch_fp(*P, ones.f(), rI, s, osNum, vc, st, nullptr, nullptr, nullptr))) return rc;
+    if ((rc = fp_any(*P, ones.f(), rI, s, osNum, vc, st, nullptr, nullptr, nullptr))) return rc;
     if ((rc = launch_inv_guard(rI, rI, frame * vc, st))) return rc;
     float* cI = colInv.f() + NV * s;
     if ((rc = bp_any(*P, ones.f(), cI, s, osNum, vc, bp_kind, st, nullptr, 0))) return rc;
@@ -541,7 +582,7 @@ int ctr_os_sart_run(ctr_plan* plan, float* d_x, const float* d_prj, int osNum, i
       const int vc = (d.PN - s + osNum - 1) / osNum;
       if (stats) CTR_HIP(hipEventRecord(kev[4 * s + 0], st));
       // tp = (prj - A_s x) ./ RowSum (os_sart.m:24,30-31)
-      if ((rc = launch_fp(*P, d_x, tp.f(), s, osNum, vc, st, d_prj, rowInv.f() + rowOff[s] * frame,
+      if ((rc = fp_any(*P, d_x, tp.f(), s, osNum, vc, st, d_prj, rowInv.f() + rowOff[s] * frame,
                           P->partials)))
         return rc;
       if (stats) CTR_HIP(hipEventRecord(kev[4 * s + 1], st));
@@ -642,7 +683,7 @@ int ctr_conjgrad(ctr_plan* plan, float* d_x, const float* d_prj, int iterNum, in
   if ((rc = ensure_partials(*P, 2 * (size_t)nparts + 2))) return rc;
   hipStream_t st = nullptr;
   auto A = [&](const float* in, float* out) -> int {  // Back(Proj(.)) (conjgrad.m:4-6)
-    int e = launch_fp(*P, in, sino.f(), 0, 1, d.PN, st, nullptr, nullptr, nullptr);
+    int e = fp_any(*P, in, sino.f(), 0, 1, d.PN, st, nullptr, nullptr, nullptr);
     if (e) return e;
     return bp_any(*P, sino.f(), out, 0, 1, d.PN, bp_kind, st, nullptr, 0);
   };

@@ -97,6 +97,12 @@ class Plan:
         check(lib().ctr_plan_create(ctypes.byref(cg), device, ctypes.byref(h)))
         self.handle = h.value
 
+    def set_method(self, method):
+        """'direct' (footprint integration, default) or 'sat' (compensated integral images)."""
+        code = {"direct": _lib.CTR_METHOD_DIRECT, "sat": _lib.CTR_METHOD_SAT}[method]
+        check(lib().ctr_plan_set_method(self.handle, code))
+        self.method = method
+
     def close(self):
         if getattr(self, "handle", None):
             lib().ctr_plan_destroy(self.handle)

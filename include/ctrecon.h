@@ -35,6 +35,14 @@ extern "C" {
 #define CTR_BP_REFERENCE 0
 #define CTR_BP_MATCHED 1
 
+/* Projector method of a plan (build addition).  DIRECT integrates each footprint
+ * over the cells it overlaps; SAT samples compensated (double-float) integral
+ * images with clamped bilinear weights, the reference's formulation
+ * (iterativeRecon/DD3_GPU.cu:134-197, 844-990, 1284-1323, 1397-1475).  Both compute
+ * the same operator; the matched adjoint is always DIRECT. */
+#define CTR_METHOD_DIRECT 0
+#define CTR_METHOD_SAT 1
+
 /*
  * Forward projection, host buffers in and out.
  * Replaces DD3Proj_gpu (iterativeRecon/DD3_GPU.cu:1132-1142), same parameter list.
@@ -88,6 +96,11 @@ typedef struct ctr_plan ctr_plan;
  * (iterativeRecon/DD3_GPU.cu:1012-1079, 1664-1686), done once per scan. */
 int ctr_plan_create(const ctr_geometry* geom, int device, ctr_plan** out);
 int ctr_plan_destroy(ctr_plan* plan);
+
+/* Select the projector method (CTR_METHOD_DIRECT, the default, or CTR_METHOD_SAT)
+ * used by every later call on the plan.  No reference counterpart: the reference
+ * always builds integral images (iterativeRecon/DD3_GPU.cu:1036, 1710). */
+int ctr_plan_set_method(ctr_plan* plan, int method);
 
 /* Forward-project device volume d_vol into device sinogram d_prj for the views
  * view_begin + k*view_stride, k < view_count; d_prj is [view_count][DNU][DNV].

@@ -1,0 +1,200 @@
+// Drives the DD3_GPU gateway through the stand-in MATLAB API.  TEST INFRASTRUCTURE.
+//   mex_driver errors        -> one line per bad call: the error identifier raised
+//   mex_driver run <dir>     -> reads geometry/inputs written by the test, calls
+//                               'Proj', 'Back' and 'OSSART', writes the outputs
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "mex.h"
+
+static mxArray* f32(float v) {
+  const mwSize d[2] = {1, 1};
+  mxArray* a = mxCreateNumericArray(2, d, mxSINGLE_CLASS, mxREAL);
+  *(float*)mxGetData(a) = v;
+  return a;
+}
+static mxArray* i32(int v) {
+  const mwSize d[2] = {1, 1};
+  mxArray* a = mxCreateNumericArray(2, d, mxINT32_CLASS, mxREAL);
+  *(int*)mxGetData(a) = v;
+  return a;
+}
+static mxArray* f64(double v) {
+  mxArray* a = mxCreateDoubleMatrix(1, 1, mxREAL);
+  *mxGetPr(a) = v;
+  return a;
+}
+static mxArray* vecf(const std::vector<float>& v) {
+  const mwSize d[2] = {1, v.size()};
+  mxArray* a = mxCreateNumericArray(2, d, mxSINGLE_CLASS, mxREAL);
+  std::memcpy(mxGetData(a), v.data(), v.size() * 4);
+  return a;
+}
+static mxArray* veci8(const std::vector<unsigned char>& v) {
+  const mwSize d[2] = {1, v.size()};
+  mxArray* a = mxCreateNumericArray(2, d, mxINT8_CLASS, mxREAL);
+  std::memcpy(mxGetData(a), v.data(), v.size());
+  return a;
+}
+
+template <typename T>
+static std::vector<T> readv(const std::string& path) {
+  std::ifstream f(path, std::ios::binary | std::ios::ate);
+  const size_t n = (size_t)f.tellg() / sizeof(T);
+  f.seekg(0);
+  std::vector<T> v(n);
+  f.read((char*)v.data(), n * sizeof(T));
+  return v;
+}
+static void writev(const std::string& path, const void* p, size_t bytes) {
+  std::ofstream f(path, std::ios::binary);
+  f.write((const char*)p, bytes);
+}
+
+struct Call {
+  std::vector<mxArray*> rhs;
+  ~Call() {
+    for (auto a : rhs) mxDestroyArray(a);
+  }
+};
+
+// rhs[0..17] for a geometry read from dir (or a 2x2x2 dummy)
+static void geometry(Call& c, const char* action, const std::string& dir) {
+  std::vector<int> n = dir.empty() ? std::vector<int>{2, 2, 2, 2, 2, 2}
+                                   : readv<int>(dir + "/sizes.i32");  // DNU DNV PN XN YN ZN
+  std::vector<float> sc = dir.empty() ? std::vector<float>{0, 100, 0, 0, 0, 0, 1, 1}
+                                      : readv<float>(dir + "/scalars.f32");
+  auto arr = [&](const char* name, int len) {
+    return dir.empty() ? std::vector<float>(len, 0.f) : readv<float>(dir + "/" + name);
+  };
+  c.rhs.push_back(mxCreateString(action));
+  c.rhs.push_back(f32(sc[0]));
+  c.rhs.push_back(f32(sc[1]));
+  c.rhs.push_back(f32(sc[2]));
+  c.rhs.push_back(i32(n[0]));
+  c.rhs.push_back(i32(n[1]));
+  c.rhs.push_back(vecf(arr("xds.f32", n[0])));
+  c.rhs.push_back(vecf(arr("yds.f32", n[0])));
+  c.rhs.push_back(vecf(arr("zds.f32", n[1])));
+  c.rhs.push_back(f32(sc[3]));
+  c.rhs.push_back(f32(sc[4]));
+  c.rhs.push_back(f32(sc[5]));
+  c.rhs.push_back(vecf(arr("angles.f32", n[2])));
+  c.rhs.push_back(vecf(arr("zshifts.f32", n[2])));
+  c.rhs.push_back(i32(n[2]));
+  c.rhs.push_back(i32(n[3]));
+  c.rhs.push_back(i32(n[4]));
+  c.rhs.push_back(i32(n[5]));
+}
+
+static std::string run(Call& c, int nlhs, mxArray** out) {
+  try {
+    mexFunction(nlhs, out, (int)c.rhs.size(), (const mxArray**)c.rhs.data());
+    return "ok";
+  } catch (const MexError& e) {
+    return e.id + " " + e.what();
+  }
+}
+
+int main(int argc, char** argv) {
+  std::string mode = argc > 1 ? argv[1] : "errors";
+  mxArray* out[2] = {nullptr, nullptr};
+  if (mode == "errors") {
+    {  // no action
+      Call c;
+      c.rhs.push_back(i32(3));
+      std::cout << "noaction " << run(c, 1, out) << "\n";
+    }
+    {  // wrong arity for Proj
+      Call c;
+      geometry(c, "Proj", "");
+      std::cout << "arity " << run(c, 1, out) << "\n";
+    }
+    {  // wrong class: image as double
+      Call c;
+      geometry(c, "Proj", "");
+      c.rhs.push_back(f64(0));
+      c.rhs.push_back(f32(1));
+      c.rhs.push_back(f32(1));
+      c.rhs.push_back(veci8({1, 1, 1, 1}));
+      c.rhs.push_back(f64(0));
+      c.rhs.push_back(i32(0));
+      std::cout << "class " << run(c, 1, out) << "\n";
+    }
+    {  // unknown action
+      Call c;
+      c.rhs.push_back(mxCreateString("Spin"));
+      std::cout << "action " << run(c, 1, out) << "\n";
+    }
+    {  // unimplemented projector mode (pseudo-DD)
+      Call c;
+      geometry(c, "Proj", "");
+      c.rhs.push_back(vecf(std::vector<float>(8, 0.f)));
+      c.rhs.push_back(f32(1));
+      c.rhs.push_back(f32(1));
+      c.rhs.push_back(veci8({1, 1, 1, 1}));
+      c.rhs.push_back(f64(0));
+      c.rhs.push_back(i32(1));
+      std::cout << "mode " << run(c, 1, out) << "\n";
+    }
+    return 0;
+  }
+  const std::string dir = argv[2];
+  auto mask = readv<unsigned char>(dir + "/mask.u8");
+  auto sc = readv<float>(dir + "/scalars.f32");
+  {
+    Call c;
+    geometry(c, "Proj", dir);
+    c.rhs.push_back(vecf(readv<float>(dir + "/vol.f32")));
+    c.rhs.push_back(f32(sc[6]));
+    c.rhs.push_back(f32(sc[7]));
+    c.rhs.push_back(veci8(mask));
+    c.rhs.push_back(f64(0));  // gpu_id as dd3.m passes it (a double)
+    c.rhs.push_back(i32(0));
+    std::string r = run(c, 1, out);
+    std::cout << "Proj " << r << "\n";
+    if (r != "ok") return 1;
+    writev(dir + "/proj_out.f32", mxGetData(out[0]), mxGetNumberOfElements(out[0]) * 4);
+    mxDestroyArray(out[0]);
+  }
+  {
+    Call c;
+    geometry(c, "Back", dir);
+    c.rhs.push_back(vecf(readv<float>(dir + "/sino.f32")));
+    c.rhs.push_back(f32(sc[6]));
+    c.rhs.push_back(f32(sc[7]));
+    c.rhs.push_back(veci8(mask));
+    c.rhs.push_back(f64(0));
+    c.rhs.push_back(i32(0));
+    c.rhs.push_back(i32(0));
+    std::string r = run(c, 1, out);
+    std::cout << "Back " << r << "\n";
+    if (r != "ok") return 1;
+    writev(dir + "/back_out.f32", mxGetData(out[0]), mxGetNumberOfElements(out[0]) * 4);
+    mxDestroyArray(out[0]);
+  }
+  {
+    Call c;
+    geometry(c, "OSSART", dir);
+    c.rhs.push_back(vecf(std::vector<float>(readv<float>(dir + "/vol.f32").size(), 0.f)));
+    c.rhs.push_back(vecf(readv<float>(dir + "/sino.f32")));
+    c.rhs.push_back(f32(sc[6]));
+    c.rhs.push_back(f32(sc[7]));
+    c.rhs.push_back(veci8(mask));
+    c.rhs.push_back(f64(0));
+    c.rhs.push_back(i32(4));  // osNum
+    c.rhs.push_back(i32(2));  // iterNum
+    c.rhs.push_back(i32(0));  // reference back projector
+    std::string r = run(c, 2, out);
+    std::cout << "OSSART " << r << "\n";
+    if (r != "ok") return 1;
+    writev(dir + "/sart_out.f32", mxGetData(out[0]), mxGetNumberOfElements(out[0]) * 4);
+    std::cout << "resid " << mxGetPr(out[1])[0] << " " << mxGetPr(out[1])[1] << "\n";
+    mxDestroyArray(out[0]);
+    mxDestroyArray(out[1]);
+  }
+  return 0;
+}
