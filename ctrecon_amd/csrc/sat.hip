@@ -151,10 +151,12 @@ __global__ __launch_bounds__(kBlock) void fp_sat_kernel(
       const float r = fmaf(fi, ct.y, ct.x);
       const double tD = fmaf(r, Bv, vp.az) + 1.0, tU = fmaf(r, Bv1, vp.az) + 1.0;
       const int base = ii * H * W;
-      const double diff = sat_sample(Hi, Lo, base, W, H, tD, tl) +
-                          sat_sample(Hi, Lo, base, W, H, tU, tr) -
-                          (sat_sample(Hi, Lo, base, W, H, tD, tr) +
-                           sat_sample(Hi, Lo, base, W, H, tU, tl));
+      // grouped as two differences so that samples clamped to the same texels
+      // cancel exactly (a footprint outside the volume gives exactly 0)
+      const double diff = (sat_sample(Hi, Lo, base, W, H, tU, tr) -
+                           sat_sample(Hi, Lo, base, W, H, tD, tr)) -
+                          (sat_sample(Hi, Lo, base, W, H, tU, tl) -
+                           sat_sample(Hi, Lo, base, W, H, tD, tl));
       acc += diff / ((tr - tl) * (tU - tD));
     }
     const float val = (float)(acc * (double)(ct.w * __builtin_amdgcn_sqrtf(ct.z + h2[v])));
@@ -233,8 +235,10 @@ __global__ __launch_bounds__(kBlock) void bp_sat_kernel(
     const double iL = (alpha - dA) * c64.ib + c64.cU + 1.0;
     const double iR = (alpha + dA) * c64.ib + c64.cU + 1.0;
     const int base = k * W * H;
-    acc += (-sat_sample(Hi, Lo, base, W, H, iD, iR) - sat_sample(Hi, Lo, base, W, H, iU, iL) +
-            sat_sample(Hi, Lo, base, W, H, iD, iL) + sat_sample(Hi, Lo, base, W, H, iU, iR)) *
+    // grouped as two differences so that a shadow outside the detector gives exactly
+    // 0: the OS-SART column-sum guard (os_sart.m:38-39) relies on exact zeros
+    acc += ((sat_sample(Hi, Lo, base, W, H, iU, iR) - sat_sample(Hi, Lo, base, W, H, iD, iR)) -
+            (sat_sample(Hi, Lo, base, W, H, iU, iL) - sat_sample(Hi, Lo, base, W, H, iD, iL))) *
            cosVal;
   }
   if constexpr (UPDATE)
