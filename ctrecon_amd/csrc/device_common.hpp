@@ -44,6 +44,27 @@ __device__ __forceinline__ int xcd_logical_block(int hw, int G) {
   return (xcd < r) ? xcd * (q + 1) + loc : r * (q + 1) + (xcd - r) * q + loc;
 }
 
+// asin for the fan angles of the reference back projector.  On |x| <= 0.5 (fan
+// half-angles up to 30 degrees): x + x^3/6 in FP64 plus the remainder x^5 Q(x^2) in
+// FP32, Q of degree 5 fitted by Chebyshev least squares to
+// (asin(x) - x - x^3/6) / x^5 on [0, 0.25]; the remainder is below 3e-3, so the
+// result is within 3.4e-10 of asin (FP32 coefficients take no registers on gfx9,
+// FP64 ones would).  Wider arguments take the library asin.  The argument is
+// uniform across a wave where it is used, so the branch does not diverge.
+__device__ __forceinline__ double asin_fan(double x) {
+  if (fabs(x) > 0.5) return asin(x);
+  const double t = x * x;
+  const double x3 = x * t;
+  const float tf = (float)t;
+  float q = 0.027716955170035362f;
+  q = fmaf(q, tf, 0.012915842235088348f);
+  q = fmaf(q, tf, 0.022988272830843925f);
+  q = fmaf(q, tf, 0.030344299972057343f);
+  q = fmaf(q, tf, 0.04464368522167206f);
+  q = fmaf(q, tf, 0.07499999552965164f);
+  return fma(x3, 1.0 / 6.0, x) + (double)((float)(x3 * t) * q);
+}
+
 // Double-float (hi + lo) value: an FP32 pair carrying about 48 bits.
 struct DF {
   float hi, lo;

@@ -54,7 +54,7 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* partials,
 // invRow by the launch's compact view index, NULL for 1) and per-block sums of
 // (b - A x)^2 and b^2.
 // ---------------------------------------------------------------------------
-template <bool FUSED>
+template <bool FUSED, bool COLW>
 __global__ __launch_bounds__(kBlock) void fp_kernel(
     const float* __restrict__ vol, float* __restrict__ out, const ViewP* __restrict__ views,
     const float2* __restrict__ bnd, const float4* __restrict__ cols,
@@ -68,10 +68,14 @@ __global__ __launch_bounds__(kBlock) void fp_kernel(
   const int k = lb / gridDim.x;
   const int p = vb + k * vs;
   const int nray = d.DNU * d.DNV;
-  const int ray = (lb - k * gridDim.x) * kBlock + threadIdx.x;
+  // COLW (DNV == 64): a wave is exactly one detector column, so everything in-plane
+  // is uniform and lives in scalar registers
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ray = COLW ? ((lb - k * gridDim.x) * kWaves + wid) * 64 + (threadIdx.x & 63)
+                       : (lb - k * gridDim.x) * kBlock + threadIdx.x;
   double res = 0.0, bsq = 0.0;
   if (ray < nray) {
-    const int u = ray / d.DNV;
+    const int u = COLW ? ((lb - k * gridDim.x) * kWaves + wid) : ray / d.DNV;
     const int v = ray - u * d.DNV;
     const ViewP vp = views[p];
     const float2 bl = bnd[(size_t)p * (d.DNU + 1) + u];
@@ -179,9 +183,10 @@ __global__ __launch_bounds__(kBlock) void bp_ref_kernel(
     const double2* __restrict__ src64, const uint8_t* __restrict__ mask, Dims d, BpConsts c,
     BpConsts64 c64, int vb, int vs, int vc, const float* __restrict__ invCol) {
   const int lane = threadIdx.x & 63;
-  const int col = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int col = blockIdx.x * kWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (col >= d.XN * d.YN) return;
-  const int zb = blockIdx.y * (64 * ZG) + lane;
+  const int zb0 = blockIdx.y * (64 * ZG);  // first z of the wave (uniform)
+  const int zb = zb0 + lane;
   const int y = col / d.XN, x = col - y * d.XN;
   float* vcol = vol + (size_t)col * d.ZN;
   if (mask[col] != 1) {
@@ -195,14 +200,14 @@ __global__ __launch_bounds__(kBlock) void bp_ref_kernel(
   const double Px = ((double)x - c64.ocx) * c64.dx;
   const double Py = ((double)y - c64.ocy) * c64.dx;
   const float Pxf = (float)Px, Pyf = (float)Py;
-  float Pz[ZG], acc[ZG];
-  bool live[ZG];
+  const float Pz0 = ((float)zb - d.ocz) * d.dz;        // this lane, group 0
+  const float dzg = 64.f * d.dz;                      // z step between groups
+  // z extent of the wave's voxel centres (uniform), for the whole-column view test
+  const float Zlo = ((float)zb0 - d.ocz) * d.dz;
+  const float Zhi = ((float)min(zb0 + 64 * ZG, d.ZN) - 1.f - d.ocz) * d.dz;
+  float acc[ZG];
 #pragma unroll
-  for (int g = 0; g < ZG; ++g) {
-    Pz[g] = ((float)(zb + 64 * g) - d.ocz) * d.dz;
-    acc[g] = 0.f;
-    live[g] = zb + 64 * g < d.ZN;
-  }
+  for (int g = 0; g < ZG; ++g) acc[g] = 0.f;
   const double DNUh = (double)d.DNU - 0.5;
   const float DNVh = (float)d.DNV - 0.5f;
   const int DNV = d.DNV;
@@ -210,31 +215,23 @@ __global__ __launch_bounds__(kBlock) void bp_ref_kernel(
   for (int k = 0; k < vc; ++k) {
     const int p = vb + k * vs;
     const ViewP vp = views[p];
-    // row shadow of every group, FP32 (DD3_GPU.cu:1446-1450)
+    // row shadow, FP32 (DD3_GPU.cu:1446-1450): t = (dZ +- hz) * S / l + cV; first the
+    // whole column's row range (uniform), to skip views it does not meet
     const float dXf = Pxf - vp.sx, dYf = Pyf - vp.sy;
     const float l2f = dXf * dXf + dYf * dYf;
     const float rlS = __builtin_amdgcn_rsqf(l2f) * c.S;
-    float vlo[ZG], vhi[ZG];
-    bool vok[ZG], vneg[ZG];
-    bool any = false;
-#pragma unroll
-    for (int g = 0; g < ZG; ++g) {
-      const float dZ = Pz[g] - vp.sz;
-      const float tU = fmaf(dZ + c.hz, rlS, c.cV);
-      const float tD = fmaf(dZ - c.hz, rlS, c.cV);
-      vneg[g] = tU < tD;
-      vlo[g] = fmaxf(fminf(tU, tD), -0.5f);
-      vhi[g] = fminf(fmaxf(tU, tD), DNVh);
-      vok[g] = live[g] && vlo[g] < vhi[g];
-      any |= (bool)__any(vok[g]);
+    const bool rneg = rlS < 0.f;  // rows then run backwards: swap the bounds
+    {
+      const float ta = fmaf(Zlo - vp.sz - c.hz, rlS, c.cV);
+      const float tb = fmaf(Zhi - vp.sz + c.hz, rlS, c.cV);
+      if (!(fmaxf(ta, tb) > -0.5f && fminf(ta, tb) < DNVh)) continue;
     }
-    if (!any) continue;
     // fan-angle shadow, FP64 (DD3_GPU.cu:1447-1464); sy*dX - sx*dY == sy*Px - sx*Py
     const double2 s64 = src64[p];
     const double dX = Px - s64.x, dY = Py - s64.y;
     const double l2 = dX * dX + dY * dY;
     const double rl = rsqrt(l2);
-    const double alpha = asin((s64.y * Px - s64.x * Py) * c64.inv_sid * rl);
+    const double alpha = asin_fan((s64.y * Px - s64.x * Py) * c64.inv_sid * rl);
     const double w = vp.wx ? dX : dY;
     const double dA = w * (rl * rl) * (c64.dx * 0.5);
     const double tL = (alpha - dA) * c64.ib + c64.cU;
@@ -249,19 +246,24 @@ __global__ __launch_bounds__(kBlock) void bp_ref_kernel(
     const float fh = (float)(uhi - ((double)(u0 + nu - 1) - 0.5));
     const float wu0 = (nu == 1) ? (float)(uhi - ulo) : fl;
     const float cw = c.dx * __builtin_amdgcn_rcpf((float)w);
-    const bool lneg = tR < tL;
+    // sign of the reference's corner difference: (tR - tL) and (tU - tD)
+    const float sgn = ((tR < tL) != rneg) ? -1.f : 1.f;
+    const float cUp = fmaf(c.hz, rlS, c.cV), cDp = fmaf(-c.hz, rlS, c.cV);
     const BufF32 pp(prj + (size_t)k * frame, frame);
 #pragma unroll
     for (int g = 0; g < ZG; ++g) {
-      if (!__any(vok[g])) continue;
-      if (!vok[g]) continue;
-      const float dZ = Pz[g] - vp.sz;
+      const float dZ = fmaf((float)g, dzg, Pz0) - vp.sz;
+      const float tU = fmaf(dZ, rlS, cUp), tD = fmaf(dZ, rlS, cDp);
+      const float vlo = fmaxf(rneg ? tU : tD, -0.5f);
+      const float vhi = fminf(rneg ? tD : tU, DNVh);
+      const bool vok = zb + 64 * g < d.ZN && vlo < vhi;
+      if (!__any(vok)) continue;
+      if (!vok) continue;
       const float cosVal = cw * __builtin_amdgcn_sqrtf(fmaf(dZ, dZ, l2f));
-      const float sgn = (lneg != vneg[g]) ? -1.f : 1.f;
-      const int v0 = (int)floorf(vlo[g] + 0.5f);
-      const int nv = min((int)floorf(vhi[g] + 0.5f), DNV - 1) - v0 + 1;
+      const int v0 = (int)floorf(vlo + 0.5f);
+      const int nv = min((int)floorf(vhi + 0.5f), DNV - 1) - v0 + 1;
       float wv0, wvl;
-      edge_weights(vlo[g], vhi[g], v0, nv, wv0, wvl);
+      edge_weights(vlo, vhi, v0, nv, wv0, wvl);
       float s = 0.f;
       if (nu <= 4 && nv <= 3) {
         const float wv1 = (nv == 2) ? wvl : (nv == 3 ? 1.f : 0.f);
@@ -296,8 +298,8 @@ __global__ __launch_bounds__(kBlock) void bp_ref_kernel(
   }
 #pragma unroll
   for (int g = 0; g < ZG; ++g) {
-    if (!live[g]) continue;
     const int z = zb + 64 * g;
+    if (z >= d.ZN) continue;
     if constexpr (UPDATE)
       vcol[z] = fmaf(acc[g], invCol[(size_t)col * d.ZN + z], vcol[z]);
     else
@@ -320,7 +322,7 @@ __global__ __launch_bounds__(kBlock) void bp_matched_kernel(
     const float* __restrict__ Bz, const uint8_t* __restrict__ mask, Dims d, BpConsts c,
     int vb, int vs, int vc, const float* __restrict__ invCol) {
   const int lane = threadIdx.x & 63;
-  const int col = blockIdx.x * kWaves + (threadIdx.x >> 6);
+  const int col = blockIdx.x * kWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (col >= d.XN * d.YN) return;
   const int zb = blockIdx.y * (64 * ZG) + lane;
   const int y = col / d.XN, x = col - y * d.XN;
@@ -537,15 +539,18 @@ int launch_fp(const Plan& P, const float* vol, float* out, int vb, int vs, int v
               hipStream_t st, const float* b, const float* invRow, double* partials) {
   if (vc <= 0) return CTR_OK;
   dim3 grid((P.d.DNU * P.d.DNV + kBlock - 1) / kBlock, vc);
+  const bool colw = P.d.DNV == 64;
+#define CTR_FP(F, C, B, IR, PA)                                                              \
+  hipLaunchKernelGGL((fp_kernel<F, C>), grid, dim3(kBlock), 0, st, vol, out, P.views, P.bnd, \
+                     P.cols, P.Bz, P.h2, P.invdB, P.maskf, P.mext, P.d, vb, vs, B, IR, PA)
   if (b) {
-    hipLaunchKernelGGL(fp_kernel<true>, grid, dim3(kBlock), 0, st, vol, out, P.views, P.bnd,
-                       P.cols, P.Bz, P.h2, P.invdB, P.maskf, P.mext, P.d, vb, vs, b, invRow,
-                       partials);
+    if (colw) CTR_FP(true, true, b, invRow, partials);
+    else CTR_FP(true, false, b, invRow, partials);
   } else {
-    hipLaunchKernelGGL(fp_kernel<false>, grid, dim3(kBlock), 0, st, vol, out, P.views, P.bnd,
-                       P.cols, P.Bz, P.h2, P.invdB, P.maskf, P.mext, P.d, vb, vs, nullptr,
-                       nullptr, nullptr);
+    if (colw) CTR_FP(false, true, nullptr, nullptr, nullptr);
+    else CTR_FP(false, false, nullptr, nullptr, nullptr);
   }
+#undef CTR_FP
   return check_launch("fp_kernel");
 }
 
