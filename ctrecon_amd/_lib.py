@@ -22,6 +22,8 @@ CTR_BP_REFERENCE = 0
 CTR_BP_MATCHED = 1
 CTR_METHOD_DIRECT = 0
 CTR_METHOD_SAT = 1
+CTR_STORAGE_F32 = 0
+CTR_STORAGE_BF16 = 1
 
 _NAMES = {0: "CTR_OK", 1: "CTR_ERR_ARG", 2: "CTR_ERR_HIP", 3: "CTR_ERR_UNSUPPORTED",
           4: "CTR_ERR_NOMEM", 5: "CTR_ERR_NUMERIC"}
@@ -29,7 +31,8 @@ _NAMES = {0: "CTR_OK", 1: "CTR_ERR_ARG", 2: "CTR_ERR_HIP", 3: "CTR_ERR_UNSUPPORT
 # Every symbol include/ctrecon.h declares.
 EXPORTS = [
     "ctr_dd3_proj", "ctr_dd3_back", "ctr_plan_create", "ctr_plan_destroy", "ctr_plan_set_method",
-    "ctr_plan_fp",
+    "ctr_plan_set_storage", "ctr_plan_fp", "ctr_plan_fp_bf16", "ctr_plan_bp_bf16",
+    "ctr_f32_to_bf16", "ctr_bf16_to_f32",
     "ctr_plan_bp", "ctr_os_sart", "ctr_os_sart_run", "ctr_cone_voxels", "ctr_conjgrad",
     "ctr_residual", "ctr_device_count",
     "ctr_malloc", "ctr_free", "ctr_memcpy_h2d", "ctr_memcpy_d2h", "ctr_memset",
@@ -81,7 +84,12 @@ def lib():
         L.ctr_plan_create.argtypes = [ctypes.POINTER(CtrGeometry), i, ctypes.POINTER(p)]
         L.ctr_plan_destroy.argtypes = [p]
         L.ctr_plan_set_method.argtypes = [p, i]
+        L.ctr_plan_set_storage.argtypes = [p, i]
         L.ctr_plan_fp.argtypes = [p, p, p, i, i, i, p]
+        L.ctr_plan_fp_bf16.argtypes = [p, p, p, i, i, i, p]
+        L.ctr_plan_bp_bf16.argtypes = [p, p, p, i, i, i, i, p]
+        L.ctr_f32_to_bf16.argtypes = [p, p, ctypes.c_ulonglong, p]
+        L.ctr_bf16_to_f32.argtypes = [p, p, ctypes.c_ulonglong, p]
         L.ctr_plan_bp.argtypes = [p, p, p, i, i, i, i, p]
         L.ctr_os_sart.argtypes = [p, p, p, i, i, i, p]
         L.ctr_os_sart_run.argtypes = [p, p, p, i, i, i, d, p, p, ctypes.POINTER(CtrSartStats)]

@@ -80,6 +80,8 @@ struct Plan {
   HostGeo hg;
   // projector method: CTR_METHOD_DIRECT or CTR_METHOD_SAT
   int method = CTR_METHOD_DIRECT;
+  // storage of the solvers' projector operands: CTR_STORAGE_F32 or CTR_STORAGE_BF16
+  int storage = CTR_STORAGE_F32;
   // scratch
   float* sat_vol = nullptr;         // volume integral images (hi then lo planes)
   size_t sat_vol_n = 0;
@@ -102,13 +104,20 @@ int fail(int code, const std::string& msg);
       return ::ctr::fail(CTR_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
-// kernel launchers (kernels.hip)
-int launch_fp(const Plan& P, const float* vol, float* out, int vb, int vs, int vc,
-              hipStream_t st, const float* b, const float* invRow, double* partials);
-int launch_bp(const Plan& P, const float* prj, float* vol, int vb, int vs, int vc, int kind,
-              hipStream_t st, const float* invCol, int accumulate);
-int launch_scale_matched(const Plan& P, const float* in, float* out, int vb, int vs, int vc,
+// kernel launchers (kernels.hip).  T is the storage type of the projector's input and
+// output arrays: float, or uint16_t holding BF16 (FP32 accumulation either way).
+// shadow: when not NULL, an UPDATE launch also writes the updated volume as BF16.
+template <typename T>
+int launch_fp(const Plan& P, const T* vol, T* out, int vb, int vs, int vc, hipStream_t st,
+              const float* b, const float* invRow, double* partials);
+template <typename T>
+int launch_bp(const Plan& P, const T* prj, float* vol, int vb, int vs, int vc, int kind,
+              hipStream_t st, const float* invCol, int accumulate, uint16_t* shadow);
+template <typename T>
+int launch_scale_matched(const Plan& P, const T* in, float* out, int vb, int vs, int vc,
                          hipStream_t st);
+int launch_to_bf16(uint16_t* dst, const float* src, size_t n, hipStream_t st);
+int launch_from_bf16(float* dst, const uint16_t* src, size_t n, hipStream_t st);
 int launch_fill(float* p, size_t n, float v, hipStream_t st);
 int launch_copy(float* dst, const float* src, size_t n, hipStream_t st);
 int launch_inv_guard(const float* in, float* out, size_t n, hipStream_t st);
@@ -122,10 +131,13 @@ int fp_partial_blocks(const Plan& P, int vc);
 // integral-image method (sat.hip)
 size_t sat_volume_floats(const Dims& d);
 size_t sat_sino_floats(const Dims& d, int vc);
-int launch_volume_sat(const Plan& P, const float* vol, float* sat, hipStream_t st);
-int launch_fp_sat(const Plan& P, const float* sat, float* out, int vb, int vs, int vc,
+template <typename T>
+int launch_volume_sat(const Plan& P, const T* vol, float* sat, hipStream_t st);
+template <typename TO>
+int launch_fp_sat(const Plan& P, const float* sat, TO* out, int vb, int vs, int vc,
                   hipStream_t st, const float* b, const float* invRow, double* partials);
-int launch_bp_sat(const Plan& P, const float* prj, float* sat, float* vol, int vb, int vs,
-                  int vc, hipStream_t st, const float* invCol, int accumulate);
+template <typename T>
+int launch_bp_sat(const Plan& P, const T* prj, float* sat, float* vol, int vb, int vs, int vc,
+                  hipStream_t st, const float* invCol, int accumulate, uint16_t* shadow);
 
 }  // namespace ctr

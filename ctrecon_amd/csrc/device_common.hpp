@@ -15,17 +15,44 @@ __device__ __forceinline__ float cell_w(float lo, float hi, float c) {
   return fminf(hi, c + 0.5f) - fmaxf(lo, c - 0.5f);
 }
 
-// Raw buffer view of a float array: 32-bit byte offsets, and reads past
+// BF16 storage: the upper 16 bits of an FP32 value, held as uint16_t.  Values are
+// widened to FP32 on load and rounded to nearest even on store (v_cvt_pk_bf16_f32);
+// all arithmetic is FP32 or wider.
+__device__ __forceinline__ float ld(const float* p, size_t i) { return p[i]; }
+__device__ __forceinline__ float ld(const uint16_t* p, size_t i) {
+  return __uint_as_float((uint32_t)p[i] << 16);
+}
+__device__ __forceinline__ void st(float* p, size_t i, float v) { p[i] = v; }
+__device__ __forceinline__ uint16_t to_bf16(float v) {
+  return __builtin_bit_cast(uint16_t, (__bf16)v);
+}
+__device__ __forceinline__ void st(uint16_t* p, size_t i, float v) { p[i] = to_bf16(v); }
+
+// Raw buffer view of a float or BF16 array: 32-bit byte offsets, and reads past
 // num_records return 0 instead of touching memory outside the array.
-struct BufF32 {
+template <typename T>
+struct Buf;
+template <>
+struct Buf<float> {
   __amdgpu_buffer_rsrc_t r;
-  __device__ __forceinline__ BufF32(const float* p, uint32_t n_floats)
-      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, (int)(n_floats * 4u),
+  __device__ __forceinline__ Buf(const float* p, uint32_t n)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), (short)0, (int)(n * 4u),
                                             0x00020000)) {}
   __device__ __forceinline__ float operator[](int i) const {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, i * 4, 0, 0));
   }
 };
+template <>
+struct Buf<uint16_t> {
+  __amdgpu_buffer_rsrc_t r;
+  __device__ __forceinline__ Buf(const uint16_t* p, uint32_t n)
+      : r(__builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t*>(p), (short)0, (int)(n * 2u),
+                                            0x00020000)) {}
+  __device__ __forceinline__ float operator[](int i) const {
+    return __uint_as_float((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, i * 2, 0, 0) << 16);
+  }
+};
+using BufF32 = Buf<float>;
 
 // Overlap weights of [lo, hi] (lo < hi, clipped to the array) with the cells
 // c0 = floor(lo + .5) ... c0 + n - 1: first, interior (1) and last.

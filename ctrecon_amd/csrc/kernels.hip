@@ -13,6 +13,7 @@
 // without the FP32 cancellation of whole-plane integral images and without the
 // integral-image build pass.
 #include <algorithm>
+#include <type_traits>
 
 #include "ctr_internal.hpp"
 #include "device_common.hpp"
@@ -53,10 +54,12 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* partials,
 // FUSED: out = (b - A x) * invRow (os_sart.m:24,30-31; b indexed by global view,
 // invRow by the launch's compact view index, NULL for 1) and per-block sums of
 // (b - A x)^2 and b^2.
+// T: storage type of vol and out (float, or uint16_t for BF16); b and invRow are FP32
+// and the ray sum is accumulated in FP32.
 // ---------------------------------------------------------------------------
-template <bool FUSED, bool COLW>
+template <bool FUSED, bool COLW, typename T>
 __global__ __launch_bounds__(kBlock) void fp_kernel(
-    const float* __restrict__ vol, float* __restrict__ out, const ViewP* __restrict__ views,
+    const T* __restrict__ vol, T* __restrict__ out, const ViewP* __restrict__ views,
     const float2* __restrict__ bnd, const float4* __restrict__ cols,
     const float* __restrict__ Bz, const float* __restrict__ h2,
     const float* __restrict__ invdB, const float* __restrict__ maskf,
@@ -91,7 +94,7 @@ __global__ __launch_bounds__(kBlock) void fp_kernel(
     const float ZNh = (float)d.ZN - 0.5f;
     const int ZN1 = d.ZN - 1;
     const float2* me = mext + (vp.xmarch ? 0 : d.XN);
-    const BufF32 V(vol, (uint32_t)(d.XN * d.YN * d.ZN));
+    const Buf<T> V(vol, (uint32_t)(d.XN * d.YN * d.ZN));
     float acc = 0.f;
     for (int ii = 0; ii < NS; ++ii) {
       const float fi = (float)ii;
@@ -156,11 +159,11 @@ __global__ __launch_bounds__(kBlock) void fp_kernel(
       const size_t og = ((size_t)p * d.DNU + u) * d.DNV + v;
       const float bb = b[og];
       const float diff = bb - val;
-      out[o] = invRow ? diff * invRow[o] : diff;
+      st(out, o, invRow ? diff * invRow[o] : diff);
       res = (double)diff * (double)diff;
       bsq = (double)bb * (double)bb;
     } else {
-      out[o] = val;
+      st(out, o, val);
     }
   }
   if constexpr (FUSED) block_sum2(res, bsq, partials, hw);
@@ -175,13 +178,16 @@ __global__ __launch_bounds__(kBlock) void fp_kernel(
 // the same for the whole column and is resolved in FP64 once per (column, view);
 // the row coordinates are per lane in FP32.  A view whose row shadow misses the
 // detector for every lane of a group is skipped for that group.
-// UPDATE: vol += acc * invCol (os_sart.m:32,37-40); otherwise vol = acc.
+// UPDATE: vol += acc * invCol (os_sart.m:32,37-40), and shadow (when not NULL) gets
+// the updated value rounded to BF16; otherwise vol = acc.
+// T: storage type of prj (float, or uint16_t for BF16); vol is FP32.
 // ---------------------------------------------------------------------------
-template <bool UPDATE, int ZG>
+template <bool UPDATE, int ZG, typename T>
 __global__ __launch_bounds__(kBlock) void bp_ref_kernel(
-    const float* __restrict__ prj, float* __restrict__ vol, const ViewP* __restrict__ views,
+    const T* __restrict__ prj, float* __restrict__ vol, const ViewP* __restrict__ views,
     const double2* __restrict__ src64, const uint8_t* __restrict__ mask, Dims d, BpConsts c,
-    BpConsts64 c64, int vb, int vs, int vc, const float* __restrict__ invCol) {
+    BpConsts64 c64, int vb, int vs, int vc, const float* __restrict__ invCol,
+    uint16_t* __restrict__ shadow) {
   const int lane = threadIdx.x & 63;
   const int col = blockIdx.x * kWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (col >= d.XN * d.YN) return;
@@ -249,7 +255,7 @@ __global__ __launch_bounds__(kBlock) void bp_ref_kernel(
     // sign of the reference's corner difference: (tR - tL) and (tU - tD)
     const float sgn = ((tR < tL) != rneg) ? -1.f : 1.f;
     const float cUp = fmaf(c.hz, rlS, c.cV), cDp = fmaf(-c.hz, rlS, c.cV);
-    const BufF32 pp(prj + (size_t)k * frame, frame);
+    const Buf<T> pp(prj + (size_t)k * frame, frame);
 #pragma unroll
     for (int g = 0; g < ZG; ++g) {
       const float dZ = fmaf((float)g, dzg, Pz0) - vp.sz;
@@ -300,10 +306,13 @@ __global__ __launch_bounds__(kBlock) void bp_ref_kernel(
   for (int g = 0; g < ZG; ++g) {
     const int z = zb + 64 * g;
     if (z >= d.ZN) continue;
-    if constexpr (UPDATE)
-      vcol[z] = fmaf(acc[g], invCol[(size_t)col * d.ZN + z], vcol[z]);
-    else
+    if constexpr (UPDATE) {
+      const float nx = fmaf(acc[g], invCol[(size_t)col * d.ZN + z], vcol[z]);
+      vcol[z] = nx;
+      if (shadow) shadow[(size_t)col * d.ZN + z] = to_bf16(nx);
+    } else {
       vcol[z] = acc[g];
+    }
   }
 }
 
@@ -320,7 +329,7 @@ __global__ __launch_bounds__(kBlock) void bp_matched_kernel(
     const float* __restrict__ yp, float* __restrict__ vol, const ViewP* __restrict__ views,
     const float2* __restrict__ bnd, const float4* __restrict__ cols,
     const float* __restrict__ Bz, const uint8_t* __restrict__ mask, Dims d, BpConsts c,
-    int vb, int vs, int vc, const float* __restrict__ invCol) {
+    int vb, int vs, int vc, const float* __restrict__ invCol, uint16_t* __restrict__ shadow) {
   const int lane = threadIdx.x & 63;
   const int col = blockIdx.x * kWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (col >= d.XN * d.YN) return;
@@ -429,16 +438,21 @@ __global__ __launch_bounds__(kBlock) void bp_matched_kernel(
     if (!live[g]) continue;
     const int z = zb + 64 * g;
     const float a = acc[g] * m;
-    if constexpr (UPDATE)
-      vcol[z] = fmaf(a, invCol[(size_t)col * d.ZN + z], vcol[z]);
-    else
+    if constexpr (UPDATE) {
+      const float nx = fmaf(a, invCol[(size_t)col * d.ZN + z], vcol[z]);
+      vcol[z] = nx;
+      if (shadow) shadow[(size_t)col * d.ZN + z] = to_bf16(nx);
+    } else {
       vcol[z] = a;
+    }
   }
 }
 
-// yp[k][u][v] = y[k][u][v] * K(p,u,v) * invdB[v]: the per-ray factors of fp_kernel.
+// yp[k][u][v] = y[k][u][v] * K(p,u,v) * invdB[v]: the per-ray factors of fp_kernel
+// (y stored as float or BF16, yp FP32).
+template <typename T>
 __global__ __launch_bounds__(kBlock) void scale_matched_kernel(
-    const float* __restrict__ in, float* __restrict__ out, const float4* __restrict__ cols,
+    const T* __restrict__ in, float* __restrict__ out, const float4* __restrict__ cols,
     const float* __restrict__ h2, const float* __restrict__ invdB, Dims d, int vb, int vs) {
   const int k = blockIdx.y;
   const int p = vb + k * vs;
@@ -448,7 +462,7 @@ __global__ __launch_bounds__(kBlock) void scale_matched_kernel(
   const float4 ct = cols[(size_t)p * d.DNU + u];
   const float K = ct.w * sqrtf(ct.z + h2[v]) * invdB[v];
   const size_t o = (size_t)k * d.DNU * d.DNV + ray;
-  out[o] = in[o] * K;
+  out[o] = ld(in, o) * K;
 }
 
 __global__ void copy_kernel(float* __restrict__ dst, const float* __restrict__ src, size_t n) {
@@ -461,6 +475,20 @@ __global__ void fill_kernel(float* __restrict__ p, size_t n, float v) {
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
     p[i] = v;
+}
+
+__global__ void to_bf16_kernel(uint16_t* __restrict__ dst, const float* __restrict__ src,
+                               size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = to_bf16(src[i]);
+}
+
+__global__ void from_bf16_kernel(float* __restrict__ dst, const uint16_t* __restrict__ src,
+                                 size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = ld(src, i);
 }
 
 // out = (in != 0) ? 1/in : 1  -- the guarded divisions of os_sart.m:30-31,38-39.
@@ -535,13 +563,14 @@ int fp_partial_blocks(const Plan& P, int vc) {
   return ((P.d.DNU * P.d.DNV + kBlock - 1) / kBlock) * vc;
 }
 
-int launch_fp(const Plan& P, const float* vol, float* out, int vb, int vs, int vc,
-              hipStream_t st, const float* b, const float* invRow, double* partials) {
+template <typename T>
+int launch_fp(const Plan& P, const T* vol, T* out, int vb, int vs, int vc, hipStream_t st,
+              const float* b, const float* invRow, double* partials) {
   if (vc <= 0) return CTR_OK;
   dim3 grid((P.d.DNU * P.d.DNV + kBlock - 1) / kBlock, vc);
   const bool colw = P.d.DNV == 64;
-#define CTR_FP(F, C, B, IR, PA)                                                              \
-  hipLaunchKernelGGL((fp_kernel<F, C>), grid, dim3(kBlock), 0, st, vol, out, P.views, P.bnd, \
+#define CTR_FP(F, C, B, IR, PA)                                                                 \
+  hipLaunchKernelGGL((fp_kernel<F, C, T>), grid, dim3(kBlock), 0, st, vol, out, P.views, P.bnd, \
                      P.cols, P.Bz, P.h2, P.invdB, P.maskf, P.mext, P.d, vb, vs, B, IR, PA)
   if (b) {
     if (colw) CTR_FP(true, true, b, invRow, partials);
@@ -554,61 +583,90 @@ int launch_fp(const Plan& P, const float* vol, float* out, int vb, int vs, int v
   return check_launch("fp_kernel");
 }
 
-int launch_bp(const Plan& P, const float* prj, float* vol, int vb, int vs, int vc, int kind,
-              hipStream_t st, const float* invCol, int accumulate) {
+template <typename T>
+int launch_bp(const Plan& P, const T* prj, float* vol, int vb, int vs, int vc, int kind,
+              hipStream_t st, const float* invCol, int accumulate, uint16_t* shadow) {
   const int zg = std::min(4, (P.d.ZN + 63) / 64);
   dim3 rgrid((P.d.XN * P.d.YN + kWaves - 1) / kWaves, (P.d.ZN + 64 * zg - 1) / (64 * zg));
   if (kind == CTR_BP_MATCHED) {
-#define CTR_BPM(UPD, ZGV, IC)                                                                  \
-  hipLaunchKernelGGL((bp_matched_kernel<UPD, ZGV>), rgrid, dim3(kBlock), 0, st, prj, vol, P.views, \
-                     P.bnd, P.cols, P.Bz, P.mask, P.d, P.bc, vb, vs, vc, IC)
-    if (accumulate) {
-      switch (zg) {
-        case 1: CTR_BPM(true, 1, invCol); break;
-        case 2: CTR_BPM(true, 2, invCol); break;
-        case 3: CTR_BPM(true, 3, invCol); break;
-        default: CTR_BPM(true, 4, invCol); break;
-      }
+    if constexpr (!std::is_same<T, float>::value) {
+      return fail(CTR_ERR_ARG, "the matched back projector reads the FP32 scaled sinogram");
     } else {
-      switch (zg) {
-        case 1: CTR_BPM(false, 1, nullptr); break;
-        case 2: CTR_BPM(false, 2, nullptr); break;
-        case 3: CTR_BPM(false, 3, nullptr); break;
-        default: CTR_BPM(false, 4, nullptr); break;
+#define CTR_BPM(UPD, ZGV, IC, SH)                                                                \
+  hipLaunchKernelGGL((bp_matched_kernel<UPD, ZGV>), rgrid, dim3(kBlock), 0, st, prj, vol, P.views, \
+                     P.bnd, P.cols, P.Bz, P.mask, P.d, P.bc, vb, vs, vc, IC, SH)
+      if (accumulate) {
+        switch (zg) {
+          case 1: CTR_BPM(true, 1, invCol, shadow); break;
+          case 2: CTR_BPM(true, 2, invCol, shadow); break;
+          case 3: CTR_BPM(true, 3, invCol, shadow); break;
+          default: CTR_BPM(true, 4, invCol, shadow); break;
+        }
+      } else {
+        switch (zg) {
+          case 1: CTR_BPM(false, 1, nullptr, nullptr); break;
+          case 2: CTR_BPM(false, 2, nullptr, nullptr); break;
+          case 3: CTR_BPM(false, 3, nullptr, nullptr); break;
+          default: CTR_BPM(false, 4, nullptr, nullptr); break;
+        }
       }
-    }
 #undef CTR_BPM
-    return check_launch("bp_matched_kernel");
+      return check_launch("bp_matched_kernel");
+    }
   }
-#define CTR_BPREF(UPD, ZGV, IC)                                                             \
-  hipLaunchKernelGGL((bp_ref_kernel<UPD, ZGV>), rgrid, dim3(kBlock), 0, st, prj, vol, P.views, \
-                     P.src64, P.mask, P.d, P.bc, P.bc64, vb, vs, vc, IC)
+#define CTR_BPREF(UPD, ZGV, IC, SH)                                                              \
+  hipLaunchKernelGGL((bp_ref_kernel<UPD, ZGV, T>), rgrid, dim3(kBlock), 0, st, prj, vol, P.views, \
+                     P.src64, P.mask, P.d, P.bc, P.bc64, vb, vs, vc, IC, SH)
   if (accumulate) {
     switch (zg) {
-      case 1: CTR_BPREF(true, 1, invCol); break;
-      case 2: CTR_BPREF(true, 2, invCol); break;
-      case 3: CTR_BPREF(true, 3, invCol); break;
-      default: CTR_BPREF(true, 4, invCol); break;
+      case 1: CTR_BPREF(true, 1, invCol, shadow); break;
+      case 2: CTR_BPREF(true, 2, invCol, shadow); break;
+      case 3: CTR_BPREF(true, 3, invCol, shadow); break;
+      default: CTR_BPREF(true, 4, invCol, shadow); break;
     }
   } else {
     switch (zg) {
-      case 1: CTR_BPREF(false, 1, nullptr); break;
-      case 2: CTR_BPREF(false, 2, nullptr); break;
-      case 3: CTR_BPREF(false, 3, nullptr); break;
-      default: CTR_BPREF(false, 4, nullptr); break;
+      case 1: CTR_BPREF(false, 1, nullptr, nullptr); break;
+      case 2: CTR_BPREF(false, 2, nullptr, nullptr); break;
+      case 3: CTR_BPREF(false, 3, nullptr, nullptr); break;
+      default: CTR_BPREF(false, 4, nullptr, nullptr); break;
     }
   }
 #undef CTR_BPREF
   return check_launch("bp_ref_kernel");
 }
 
-int launch_scale_matched(const Plan& P, const float* in, float* out, int vb, int vs, int vc,
+template <typename T>
+int launch_scale_matched(const Plan& P, const T* in, float* out, int vb, int vs, int vc,
                          hipStream_t st) {
   if (vc <= 0) return CTR_OK;
   dim3 grid((P.d.DNU * P.d.DNV + kBlock - 1) / kBlock, vc);
-  hipLaunchKernelGGL(scale_matched_kernel, grid, dim3(kBlock), 0, st, in, out, P.cols, P.h2,
+  hipLaunchKernelGGL(scale_matched_kernel<T>, grid, dim3(kBlock), 0, st, in, out, P.cols, P.h2,
                      P.invdB, P.d, vb, vs);
   return check_launch("scale_matched_kernel");
+}
+
+template int launch_fp<float>(const Plan&, const float*, float*, int, int, int, hipStream_t,
+                              const float*, const float*, double*);
+template int launch_fp<uint16_t>(const Plan&, const uint16_t*, uint16_t*, int, int, int,
+                                 hipStream_t, const float*, const float*, double*);
+template int launch_bp<float>(const Plan&, const float*, float*, int, int, int, int, hipStream_t,
+                              const float*, int, uint16_t*);
+template int launch_bp<uint16_t>(const Plan&, const uint16_t*, float*, int, int, int, int,
+                                 hipStream_t, const float*, int, uint16_t*);
+template int launch_scale_matched<float>(const Plan&, const float*, float*, int, int, int,
+                                         hipStream_t);
+template int launch_scale_matched<uint16_t>(const Plan&, const uint16_t*, float*, int, int, int,
+                                            hipStream_t);
+
+int launch_to_bf16(uint16_t* dst, const float* src, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(to_bf16_kernel, dim3(grid_1d(n)), dim3(256), 0, st, dst, src, n);
+  return check_launch("to_bf16_kernel");
+}
+
+int launch_from_bf16(float* dst, const uint16_t* src, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(from_bf16_kernel, dim3(grid_1d(n)), dim3(256), 0, st, dst, src, n);
+  return check_launch("from_bf16_kernel");
 }
 
 int launch_copy(float* dst, const float* src, size_t n, hipStream_t st) {

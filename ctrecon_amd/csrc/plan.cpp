@@ -316,34 +316,40 @@ static int ensure_buf(float** p, size_t* have, size_t need, const char* what) {
   return CTR_OK;
 }
 
-// Forward projection with the plan's method (fused residual when b != NULL).
-static int fp_any(Plan& P, const float* vol, float* out, int vb, int vs, int vc, hipStream_t st,
+// Forward projection with the plan's method (fused residual when b != NULL); T is the
+// storage type of vol and out.
+template <typename T>
+static int fp_any(Plan& P, const T* vol, T* out, int vb, int vs, int vc, hipStream_t st,
                   const float* b, const float* invRow, double* partials) {
   if (P.method == CTR_METHOD_SAT) {
     int rc = ensure_buf(&P.sat_vol, &P.sat_vol_n, 2 * sat_volume_floats(P.d),
                         "volume integral images");
     if (rc) return rc;
-    if ((rc = launch_volume_sat(P, vol, P.sat_vol, st))) return rc;
-    return launch_fp_sat(P, P.sat_vol, out, vb, vs, vc, st, b, invRow, partials);
+    if ((rc = launch_volume_sat<T>(P, vol, P.sat_vol, st))) return rc;
+    return launch_fp_sat<T>(P, P.sat_vol, out, vb, vs, vc, st, b, invRow, partials);
   }
-  return launch_fp(P, vol, out, vb, vs, vc, st, b, invRow, partials);
+  return launch_fp<T>(P, vol, out, vb, vs, vc, st, b, invRow, partials);
 }
 
-static int bp_any(Plan& P, const float* prj, float* vol, int vb, int vs, int vc, int kind,
-                  hipStream_t st, const float* invCol, int accumulate) {
+// Back projection (T: storage type of prj; vol FP32); shadow as in launch_bp.
+template <typename T>
+static int bp_any(Plan& P, const T* prj, float* vol, int vb, int vs, int vc, int kind,
+                  hipStream_t st, const float* invCol, int accumulate,
+                  uint16_t* shadow = nullptr) {
   if (kind == CTR_BP_REFERENCE && P.method == CTR_METHOD_SAT) {
     int rc = ensure_buf(&P.sat_sino, &P.sat_sino_n, 2 * sat_sino_floats(P.d, vc),
                         "sinogram integral images");
     if (rc) return rc;
-    return launch_bp_sat(P, prj, P.sat_sino, vol, vb, vs, vc, st, invCol, accumulate);
+    return launch_bp_sat<T>(P, prj, P.sat_sino, vol, vb, vs, vc, st, invCol, accumulate, shadow);
   }
   if (kind == CTR_BP_MATCHED) {
     int rc = ensure_scratch_prj(P, vc);
     if (rc) return rc;
-    if ((rc = launch_scale_matched(P, prj, P.scratch_prj, vb, vs, vc, st))) return rc;
-    return launch_bp(P, P.scratch_prj, vol, vb, vs, vc, kind, st, invCol, accumulate);
+    if ((rc = launch_scale_matched<T>(P, prj, P.scratch_prj, vb, vs, vc, st))) return rc;
+    return launch_bp<float>(P, P.scratch_prj, vol, vb, vs, vc, kind, st, invCol, accumulate,
+                            shadow);
   }
-  return launch_bp(P, prj, vol, vb, vs, vc, kind, st, invCol, accumulate);
+  return launch_bp<T>(P, prj, vol, vb, vs, vc, kind, st, invCol, accumulate, shadow);
 }
 
 // Owns a set of HIP events.
@@ -374,6 +380,7 @@ struct DBuf {
   }
   float* f() const { return (float*)p; }
   double* d() const { return (double*)p; }
+  uint16_t* h() const { return (uint16_t*)p; }
 };
 
 }  // namespace ctr
@@ -495,6 +502,51 @@ int ctr_plan_bp(ctr_plan* plan, const float* d_prj, float* d_vol, int view_begin
                 (hipStream_t)stream, nullptr, 0);
 }
 
+int ctr_plan_set_storage(ctr_plan* plan, int storage) {
+  Plan* P = get(plan);
+  if (!P) return fail(CTR_ERR_ARG, "NULL plan");
+  if (storage != CTR_STORAGE_F32 && storage != CTR_STORAGE_BF16)
+    return fail(CTR_ERR_ARG, "storage must be CTR_STORAGE_F32 or CTR_STORAGE_BF16");
+  P->storage = storage;
+  return CTR_OK;
+}
+
+int ctr_plan_fp_bf16(ctr_plan* plan, const unsigned short* d_vol, unsigned short* d_prj,
+                     int view_begin, int view_stride, int view_count, void* stream) {
+  Plan* P = get(plan);
+  if (!P || !d_vol || !d_prj) return fail(CTR_ERR_ARG, "NULL plan or buffer");
+  int rc = check_views(*P, view_begin, view_stride, view_count);
+  if (rc) return rc;
+  CTR_HIP(hipSetDevice(P->device));
+  return fp_any<uint16_t>(*P, d_vol, d_prj, view_begin, view_stride, view_count,
+                          (hipStream_t)stream, nullptr, nullptr, nullptr);
+}
+
+int ctr_plan_bp_bf16(ctr_plan* plan, const unsigned short* d_prj, float* d_vol, int view_begin,
+                     int view_stride, int view_count, int bp_kind, void* stream) {
+  Plan* P = get(plan);
+  if (!P || !d_vol || !d_prj) return fail(CTR_ERR_ARG, "NULL plan or buffer");
+  if (bp_kind != CTR_BP_REFERENCE && bp_kind != CTR_BP_MATCHED)
+    return fail(CTR_ERR_ARG, "bp_kind must be CTR_BP_REFERENCE or CTR_BP_MATCHED");
+  int rc = check_views(*P, view_begin, view_stride, view_count);
+  if (rc) return rc;
+  CTR_HIP(hipSetDevice(P->device));
+  return bp_any<uint16_t>(*P, d_prj, d_vol, view_begin, view_stride, view_count, bp_kind,
+                          (hipStream_t)stream, nullptr, 0);
+}
+
+int ctr_f32_to_bf16(unsigned short* d_dst, const float* d_src, unsigned long long n,
+                    void* stream) {
+  if (!d_dst || !d_src) return fail(CTR_ERR_ARG, "NULL pointer");
+  return launch_to_bf16(d_dst, d_src, n, (hipStream_t)stream);
+}
+
+int ctr_bf16_to_f32(float* d_dst, const unsigned short* d_src, unsigned long long n,
+                    void* stream) {
+  if (!d_dst || !d_src) return fail(CTR_ERR_ARG, "NULL pointer");
+  return launch_from_bf16(d_dst, d_src, n, (hipStream_t)stream);
+}
+
 int ctr_residual(ctr_plan* plan, const float* d_x, const float* d_prj, double* out) {
   Plan* P = get(plan);
   if (!P || !d_x || !d_prj || !out) return fail(CTR_ERR_ARG, "NULL argument");
@@ -547,7 +599,12 @@ int ctr_os_sart_run(ctr_plan* plan, float* d_x, const float* d_prj, int osNum, i
   DBuf rowInv, colInv, tp, ones, sums;
   if ((rc = rowInv.alloc(frame * d.PN * sizeof(float)))) return rc;  // subset-compact order
   if ((rc = colInv.alloc(NV * osNum * sizeof(float)))) return rc;
-  if ((rc = tp.alloc(frame * maxV * sizeof(float)))) return rc;
+  // BF16 storage: the forward projector reads the shadow xs of the FP32 iterate and
+  // writes the subset residual as BF16; the back projector updates x and xs together
+  const bool bf = P->storage == CTR_STORAGE_BF16;
+  DBuf xs;
+  if ((rc = tp.alloc(frame * maxV * (bf ? sizeof(uint16_t) : sizeof(float))))) return rc;
+  if (bf && (rc = xs.alloc(NV * sizeof(uint16_t)))) return rc;
   if ((rc = ones.alloc(std::max(NV, frame * maxV) * sizeof(float)))) return rc;
   if ((rc = sums.alloc(2 * sizeof(double) * osNum))) return rc;
   if ((rc = ensure_partials(*P, 2 * (size_t)fp_partial_blocks(*P, maxV) + 2))) return rc;
@@ -574,6 +631,7 @@ int ctr_os_sart_run(ctr_plan* plan, float* d_x, const float* d_prj, int osNum, i
     if ((rc = launch_inv_guard(cI, cI, NV, st))) return rc;
     off += vc;
   }
+  if (bf && (rc = launch_to_bf16(xs.h(), d_x, NV, st))) return rc;
   if (timing) CTR_HIP(hipEventRecord(ev.e[1], st));
   hipEvent_t* kev = timing ? ev.e.data() + 2 + maxIter : nullptr;
   int done = 0;
@@ -582,16 +640,19 @@ int ctr_os_sart_run(ctr_plan* plan, float* d_x, const float* d_prj, int osNum, i
       const int vc = (d.PN - s + osNum - 1) / osNum;
       if (stats) CTR_HIP(hipEventRecord(kev[4 * s + 0], st));
       // tp = (prj - A_s x) ./ RowSum (os_sart.m:24,30-31)
-      if ((rc = fp_any(*P, d_x, tp.f(), s, osNum, vc, st, d_prj, rowInv.f() + rowOff[s] * frame,
-                          P->partials)))
-        return rc;
+      const float* rI = rowInv.f() + rowOff[s] * frame;
+      rc = bf ? fp_any<uint16_t>(*P, xs.h(), tp.h(), s, osNum, vc, st, d_prj, rI, P->partials)
+              : fp_any<float>(*P, d_x, tp.f(), s, osNum, vc, st, d_prj, rI, P->partials);
+      if (rc) return rc;
       if (stats) CTR_HIP(hipEventRecord(kev[4 * s + 1], st));
       if ((rc = launch_reduce_pairs(P->partials, fp_partial_blocks(*P, vc), sums.d() + 2 * s, st)))
         return rc;
       if (stats) CTR_HIP(hipEventRecord(kev[4 * s + 2], st));
       // x += B_s(tp) ./ ColSum (os_sart.m:32,37-40)
-      if ((rc = bp_any(*P, tp.f(), d_x, s, osNum, vc, bp_kind, st, colInv.f() + NV * s, 1)))
-        return rc;
+      const float* cI = colInv.f() + NV * s;
+      rc = bf ? bp_any<uint16_t>(*P, tp.h(), d_x, s, osNum, vc, bp_kind, st, cI, 1, xs.h())
+              : bp_any<float>(*P, tp.f(), d_x, s, osNum, vc, bp_kind, st, cI, 1);
+      if (rc) return rc;
       if (stats) CTR_HIP(hipEventRecord(kev[4 * s + 3], st));
     }
     if (timing) CTR_HIP(hipEventRecord(ev.e[2 + it], st));
@@ -673,19 +734,31 @@ int ctr_conjgrad(ctr_plan* plan, float* d_x, const float* d_prj, int iterNum, in
   const Dims& d = P->d;
   const size_t NV = (size_t)d.XN * d.YN * d.ZN, NS = (size_t)d.DNU * d.DNV * d.PN;
   int rc;
-  DBuf r, pv, t, sino, dots;
+  // BF16 storage: A reads a BF16 copy of its input and passes a BF16 sinogram between
+  // the projectors; the CG vectors stay FP32 and the dots / scalars FP64
+  const bool bf = P->storage == CTR_STORAGE_BF16;
+  DBuf r, pv, t, sino, dots, in16;
   if ((rc = r.alloc(NV * sizeof(float)))) return rc;
   if ((rc = pv.alloc(NV * sizeof(float)))) return rc;
   if ((rc = t.alloc(NV * sizeof(float)))) return rc;
-  if ((rc = sino.alloc(NS * sizeof(float)))) return rc;
+  if ((rc = sino.alloc(NS * (bf ? sizeof(uint16_t) : sizeof(float))))) return rc;
+  if (bf && (rc = in16.alloc(NV * sizeof(uint16_t)))) return rc;
   if ((rc = dots.alloc(2 * sizeof(double)))) return rc;
   const int nparts = 1024;
   if ((rc = ensure_partials(*P, 2 * (size_t)nparts + 2))) return rc;
   hipStream_t st = nullptr;
   auto A = [&](const float* in, float* out) -> int {  // Back(Proj(.)) (conjgrad.m:4-6)
-    int e = fp_any(*P, in, sino.f(), 0, 1, d.PN, st, nullptr, nullptr, nullptr);
+    if (bf) {
+      int e = launch_to_bf16(in16.h(), in, NV, st);
+      if (e) return e;
+      if ((e = fp_any<uint16_t>(*P, in16.h(), sino.h(), 0, 1, d.PN, st, nullptr, nullptr,
+                                nullptr)))
+        return e;
+      return bp_any<uint16_t>(*P, sino.h(), out, 0, 1, d.PN, bp_kind, st, nullptr, 0);
+    }
+    int e = fp_any<float>(*P, in, sino.f(), 0, 1, d.PN, st, nullptr, nullptr, nullptr);
     if (e) return e;
-    return bp_any(*P, sino.f(), out, 0, 1, d.PN, bp_kind, st, nullptr, 0);
+    return bp_any<float>(*P, sino.f(), out, 0, 1, d.PN, bp_kind, st, nullptr, 0);
   };
   auto dot = [&](const float* a, const float* b, double* outv) -> int {
     int e = launch_dot(a, b, NV, P->partials, nparts, dots.d(), st);

@@ -21,29 +21,32 @@ namespace {
 
 // Source of the value at padded position (slice s, row r', column c'); r' = 0 and
 // c' = 0 are the zero border (DD3_GPU.cu:45-47, 1250-1251).
+template <typename T>
 struct VolumeRowsX {  // slices x, rows y', columns z' (the ZYX table)
-  const float* vol;
+  const T* vol;
   const float* maskf;
   int XN, YN, ZN;
   __device__ float operator()(int s, int r, int c) const {
     const int col = (r - 1) * XN + s;
-    return vol[(size_t)col * ZN + (c - 1)] * maskf[col];
+    return ld(vol, (size_t)col * ZN + (c - 1)) * maskf[col];
   }
 };
+template <typename T>
 struct VolumeRowsY {  // slices y, rows x', columns z' (the ZXY table)
-  const float* vol;
+  const T* vol;
   const float* maskf;
   int XN, YN, ZN;
   __device__ float operator()(int s, int r, int c) const {
     const int col = s * XN + (r - 1);
-    return vol[(size_t)col * ZN + (c - 1)] * maskf[col];
+    return ld(vol, (size_t)col * ZN + (c - 1)) * maskf[col];
   }
 };
+template <typename T>
 struct SinoRows {  // slices k (views of the launch), rows u', columns v'
-  const float* prj;
+  const T* prj;
   int DNU, DNV;
   __device__ float operator()(int s, int r, int c) const {
-    return prj[((size_t)s * DNU + (r - 1)) * DNV + (c - 1)];
+    return ld(prj, ((size_t)s * DNU + (r - 1)) * DNV + (c - 1));
   }
 };
 
@@ -118,12 +121,13 @@ __device__ __forceinline__ double sat_sample(const BufF32& Hi, const BufF32& Lo,
 }
 
 // Forward projector on the integral images (DD3_GPU.cu:844-990): per slice the four
-// corner samples of the footprint, divided by the footprint area.
-template <bool FUSED>
+// corner samples of the footprint, divided by the footprint area.  TO: storage
+// type of out (float, or uint16_t for BF16).
+template <bool FUSED, typename TO>
 __global__ __launch_bounds__(kBlock) void fp_sat_kernel(
     const float* __restrict__ zyx_hi, const float* __restrict__ zyx_lo,
     const float* __restrict__ zxy_hi, const float* __restrict__ zxy_lo,
-    float* __restrict__ out, const ViewP* __restrict__ views, const float2* __restrict__ bnd,
+    TO* __restrict__ out, const ViewP* __restrict__ views, const float2* __restrict__ bnd,
     const float4* __restrict__ cols, const float* __restrict__ Bz,
     const float* __restrict__ h2, Dims d, int vb, int vs, const float* __restrict__ b,
     const float* __restrict__ invRow, double* __restrict__ partials) {
@@ -165,11 +169,11 @@ __global__ __launch_bounds__(kBlock) void fp_sat_kernel(
       const size_t og = ((size_t)p * d.DNU + u) * d.DNV + v;
       const float bb = b[og];
       const float diffv = bb - val;
-      out[o] = invRow ? diffv * invRow[o] : diffv;
+      st(out, o, invRow ? diffv * invRow[o] : diffv);
       res = (double)diffv * (double)diffv;
       bsq = (double)bb * (double)bb;
     } else {
-      out[o] = val;
+      st(out, o, val);
     }
   }
   if constexpr (FUSED) {
@@ -196,13 +200,14 @@ __global__ __launch_bounds__(kBlock) void fp_sat_kernel(
 }
 
 // Reference back projector on the sinogram integral images (DD3_GPU.cu:1397-1475):
-// one wave per voxel column, one voxel per lane.
+// one wave per voxel column, one voxel per lane.  UPDATE also refreshes the BF16
+// shadow of vol when shadow is not NULL.
 template <bool UPDATE>
 __global__ __launch_bounds__(kBlock) void bp_sat_kernel(
     const float* __restrict__ shi, const float* __restrict__ slo, float* __restrict__ vol,
     const ViewP* __restrict__ views, const double2* __restrict__ src64,
     const uint8_t* __restrict__ mask, Dims d, BpConsts c, BpConsts64 c64, int vb, int vs,
-    int vc, const float* __restrict__ invCol) {
+    int vc, const float* __restrict__ invCol, uint16_t* __restrict__ shadow) {
   const int lane = threadIdx.x & 63;
   const int col = blockIdx.x * kWaves + (threadIdx.x >> 6);
   const int z = blockIdx.y * 64 + lane;
@@ -241,10 +246,13 @@ __global__ __launch_bounds__(kBlock) void bp_sat_kernel(
             (sat_sample(Hi, Lo, base, W, H, iU, iL) - sat_sample(Hi, Lo, base, W, H, iD, iL))) *
            cosVal;
   }
-  if constexpr (UPDATE)
-    vol[o] = fmaf((float)acc, invCol[o], vol[o]);
-  else
+  if constexpr (UPDATE) {
+    const float nx = fmaf((float)acc, invCol[o], vol[o]);
+    vol[o] = nx;
+    if (shadow) shadow[o] = to_bf16(nx);
+  } else {
     vol[o] = (float)acc;
+  }
 }
 
 int check(const char* what) {
@@ -276,20 +284,22 @@ size_t sat_sino_floats(const Dims& d, int vc) {
 }
 
 // Volume integral images into sat (hi planes, then lo planes: ZYX | ZXY each).
-int launch_volume_sat(const Plan& P, const float* vol, float* sat, hipStream_t st) {
+template <typename T>
+int launch_volume_sat(const Plan& P, const T* vol, float* sat, hipStream_t st) {
   const Dims& d = P.d;
   const size_t nx = (size_t)d.XN * (d.YN + 1) * (d.ZN + 1);
   const size_t n = sat_volume_floats(d);
   float* hi = sat;
   float* lo = sat + n;
-  int rc = build_sat(VolumeRowsX{vol, P.maskf, d.XN, d.YN, d.ZN}, d.XN, d.YN, d.ZN + 1, hi, lo,
-                     st);
+  int rc = build_sat(VolumeRowsX<T>{vol, P.maskf, d.XN, d.YN, d.ZN}, d.XN, d.YN, d.ZN + 1, hi,
+                     lo, st);
   if (rc) return rc;
-  return build_sat(VolumeRowsY{vol, P.maskf, d.XN, d.YN, d.ZN}, d.YN, d.XN, d.ZN + 1, hi + nx,
-                   lo + nx, st);
+  return build_sat(VolumeRowsY<T>{vol, P.maskf, d.XN, d.YN, d.ZN}, d.YN, d.XN, d.ZN + 1,
+                   hi + nx, lo + nx, st);
 }
 
-int launch_fp_sat(const Plan& P, const float* sat, float* out, int vb, int vs, int vc,
+template <typename TO>
+int launch_fp_sat(const Plan& P, const float* sat, TO* out, int vb, int vs, int vc,
                   hipStream_t st, const float* b, const float* invRow, double* partials) {
   const Dims& d = P.d;
   const size_t nx = (size_t)d.XN * (d.YN + 1) * (d.ZN + 1);
@@ -297,30 +307,41 @@ int launch_fp_sat(const Plan& P, const float* sat, float* out, int vb, int vs, i
   const float *zyx_hi = sat, *zxy_hi = sat + nx, *zyx_lo = sat + n, *zxy_lo = sat + n + nx;
   dim3 grid((d.DNU * d.DNV + kBlock - 1) / kBlock, vc);
   if (b)
-    hipLaunchKernelGGL(fp_sat_kernel<true>, grid, dim3(kBlock), 0, st, zyx_hi, zyx_lo, zxy_hi,
-                       zxy_lo, out, P.views, P.bnd, P.cols, P.Bz, P.h2, d, vb, vs, b, invRow,
-                       partials);
+    hipLaunchKernelGGL((fp_sat_kernel<true, TO>), grid, dim3(kBlock), 0, st, zyx_hi, zyx_lo,
+                       zxy_hi, zxy_lo, out, P.views, P.bnd, P.cols, P.Bz, P.h2, d, vb, vs, b,
+                       invRow, partials);
   else
-    hipLaunchKernelGGL(fp_sat_kernel<false>, grid, dim3(kBlock), 0, st, zyx_hi, zyx_lo, zxy_hi,
-                       zxy_lo, out, P.views, P.bnd, P.cols, P.Bz, P.h2, d, vb, vs, nullptr,
-                       nullptr, nullptr);
+    hipLaunchKernelGGL((fp_sat_kernel<false, TO>), grid, dim3(kBlock), 0, st, zyx_hi, zyx_lo,
+                       zxy_hi, zxy_lo, out, P.views, P.bnd, P.cols, P.Bz, P.h2, d, vb, vs,
+                       nullptr, nullptr, nullptr);
   return check("fp_sat_kernel");
 }
 
-int launch_bp_sat(const Plan& P, const float* prj, float* sat, float* vol, int vb, int vs,
-                  int vc, hipStream_t st, const float* invCol, int accumulate) {
+template <typename T>
+int launch_bp_sat(const Plan& P, const T* prj, float* sat, float* vol, int vb, int vs, int vc,
+                  hipStream_t st, const float* invCol, int accumulate, uint16_t* shadow) {
   const Dims& d = P.d;
   const size_t n = sat_sino_floats(d, vc);
-  int rc = build_sat(SinoRows{prj, d.DNU, d.DNV}, vc, d.DNU, d.DNV + 1, sat, sat + n, st);
+  int rc = build_sat(SinoRows<T>{prj, d.DNU, d.DNV}, vc, d.DNU, d.DNV + 1, sat, sat + n, st);
   if (rc) return rc;
   dim3 grid((d.XN * d.YN + kWaves - 1) / kWaves, (d.ZN + 63) / 64);
   if (accumulate)
     hipLaunchKernelGGL(bp_sat_kernel<true>, grid, dim3(kBlock), 0, st, sat, sat + n, vol,
-                       P.views, P.src64, P.mask, d, P.bc, P.bc64, vb, vs, vc, invCol);
+                       P.views, P.src64, P.mask, d, P.bc, P.bc64, vb, vs, vc, invCol, shadow);
   else
     hipLaunchKernelGGL(bp_sat_kernel<false>, grid, dim3(kBlock), 0, st, sat, sat + n, vol,
-                       P.views, P.src64, P.mask, d, P.bc, P.bc64, vb, vs, vc, nullptr);
+                       P.views, P.src64, P.mask, d, P.bc, P.bc64, vb, vs, vc, nullptr, nullptr);
   return check("bp_sat_kernel");
 }
+
+#define CTR_SAT_INST(T)                                                                       \
+  template int launch_volume_sat<T>(const Plan&, const T*, float*, hipStream_t);              \
+  template int launch_fp_sat<T>(const Plan&, const float*, T*, int, int, int, hipStream_t,    \
+                                const float*, const float*, double*);                         \
+  template int launch_bp_sat<T>(const Plan&, const T*, float*, float*, int, int, int,         \
+                                hipStream_t, const float*, int, uint16_t*);
+CTR_SAT_INST(float)
+CTR_SAT_INST(uint16_t)
+#undef CTR_SAT_INST
 
 }  // namespace ctr

@@ -16,7 +16,7 @@ from . import _lib
 from ._lib import CTR_BP_MATCHED, CTR_BP_REFERENCE, CtrGeometry, CtrSartStats, check, lib
 from .geometry import Geometry
 
-__all__ = ["DeviceArray", "Plan", "device_count", "bp_kind_code"]
+__all__ = ["DeviceArray", "Plan", "device_count", "bp_kind_code", "to_bf16", "from_bf16"]
 
 
 def device_count() -> int:
@@ -77,6 +77,24 @@ class DeviceArray:
             pass
 
 
+def to_bf16(d_src: DeviceArray, d_dst: DeviceArray = None, stream=None) -> DeviceArray:
+    """Round an FP32 device array to BF16 (nearest even) on the device."""
+    if d_dst is None:
+        d_dst = DeviceArray(d_src.shape, np.uint16, d_src.device)
+    n = int(np.prod(d_src.shape))
+    check(lib().ctr_f32_to_bf16(d_dst.ptr, d_src.ptr, n, stream))
+    return d_dst
+
+
+def from_bf16(d_src: DeviceArray, d_dst: DeviceArray = None, stream=None) -> DeviceArray:
+    """Widen a BF16 device array to FP32 on the device."""
+    if d_dst is None:
+        d_dst = DeviceArray(d_src.shape, np.float32, d_src.device)
+    n = int(np.prod(d_src.shape))
+    check(lib().ctr_bf16_to_f32(d_dst.ptr, d_src.ptr, n, stream))
+    return d_dst
+
+
 class Plan:
     """Geometry tables of one scan on one GPU (``ctr_plan_create``)."""
 
@@ -103,6 +121,13 @@ class Plan:
         check(lib().ctr_plan_set_method(self.handle, code))
         self.method = method
 
+    def set_storage(self, storage):
+        """Storage of the OS-SART / CG drivers' projector operands: 'f32' (default) or
+        'bf16' (FP32 master iterate, BF16 shadow and BF16 residual sinogram)."""
+        code = {"f32": _lib.CTR_STORAGE_F32, "bf16": _lib.CTR_STORAGE_BF16}[storage]
+        check(lib().ctr_plan_set_storage(self.handle, code))
+        self.storage = storage
+
     def close(self):
         if getattr(self, "handle", None):
             lib().ctr_plan_destroy(self.handle)
@@ -128,6 +153,24 @@ class Plan:
             view_count = len(range(view_begin, self.g.PN, view_stride))
         check(lib().ctr_plan_bp(self.handle, d_prj.ptr, d_vol.ptr, view_begin, view_stride,
                                 view_count, bp_kind_code(kind), stream))
+
+    def fp_bf16(self, d_vol: DeviceArray, d_prj: DeviceArray, view_begin=0, view_stride=1,
+                view_count=None, stream=None):
+        """Forward projection of a BF16 volume into a BF16 sinogram (uint16 arrays)."""
+        assert d_vol.dtype == np.uint16 and d_prj.dtype == np.uint16
+        if view_count is None:
+            view_count = len(range(view_begin, self.g.PN, view_stride))
+        check(lib().ctr_plan_fp_bf16(self.handle, d_vol.ptr, d_prj.ptr, view_begin, view_stride,
+                                     view_count, stream))
+
+    def bp_bf16(self, d_prj: DeviceArray, d_vol: DeviceArray, view_begin=0, view_stride=1,
+                view_count=None, kind="ref", stream=None):
+        """Back projection of a BF16 sinogram (uint16) into an FP32 volume."""
+        assert d_prj.dtype == np.uint16 and d_vol.dtype == np.float32
+        if view_count is None:
+            view_count = len(range(view_begin, self.g.PN, view_stride))
+        check(lib().ctr_plan_bp_bf16(self.handle, d_prj.ptr, d_vol.ptr, view_begin, view_stride,
+                                     view_count, bp_kind_code(kind), stream))
 
     def os_sart(self, d_x: DeviceArray, d_prj: DeviceArray, osNum: int, iterNum: int,
                 kind="ref"):
@@ -187,6 +230,23 @@ class Plan:
         dp = DeviceArray.from_host(np.asarray(prj, np.float32), self.device)
         dv = DeviceArray(self.g.vol_shape, np.float32, self.device)
         self.bp(dp, dv, vb, vs, vc, kind)
+        return dv.get()
+
+    def forward_bf16(self, vol, views=None):
+        """forward() with BF16 storage: vol is rounded to BF16 on the device and the
+        sinogram is stored as BF16; returns it widened to float32."""
+        vb, vs, vc = _view_range(self.g.PN, views)
+        dv = to_bf16(DeviceArray.from_host(np.asarray(vol, np.float32), self.device))
+        dp = DeviceArray((vc, self.g.DNU, self.g.DNV), np.uint16, self.device)
+        self.fp_bf16(dv, dp, vb, vs, vc)
+        return from_bf16(dp).get()
+
+    def back_bf16(self, prj, views=None, kind="ref"):
+        """back() with the sinogram rounded to BF16 on the device (FP32 output)."""
+        vb, vs, vc = _view_range(self.g.PN, views)
+        dp = to_bf16(DeviceArray.from_host(np.asarray(prj, np.float32), self.device))
+        dv = DeviceArray(self.g.vol_shape, np.float32, self.device)
+        self.bp_bf16(dp, dv, vb, vs, vc, kind)
         return dv.get()
 
 

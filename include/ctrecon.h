@@ -43,6 +43,12 @@ extern "C" {
 #define CTR_METHOD_DIRECT 0
 #define CTR_METHOD_SAT 1
 
+/* Storage of projector operands (build addition; the reference is FP32 throughout).
+ * BF16 arrays are unsigned short bit patterns: the upper half of the FP32 value,
+ * rounded to nearest even.  Every sum is accumulated in FP32 or wider. */
+#define CTR_STORAGE_F32 0
+#define CTR_STORAGE_BF16 1
+
 /*
  * Forward projection, host buffers in and out.
  * Replaces DD3Proj_gpu (iterativeRecon/DD3_GPU.cu:1132-1142), same parameter list.
@@ -102,6 +108,14 @@ int ctr_plan_destroy(ctr_plan* plan);
  * always builds integral images (iterativeRecon/DD3_GPU.cu:1036, 1710). */
 int ctr_plan_set_method(ctr_plan* plan, int method);
 
+/* Storage the OS-SART and CG drivers use for their projector operands (build
+ * addition, no reference counterpart).  CTR_STORAGE_BF16: the iterate keeps its FP32
+ * master copy (d_x) and a BF16 shadow that the forward projector reads; the sinogram
+ * passed from the forward to the back projector is BF16; the measured sinogram d_prj,
+ * the row / column sums, the update and every reduction stay FP32 / FP64.
+ * ctr_residual always projects in FP32. */
+int ctr_plan_set_storage(ctr_plan* plan, int storage);
+
 /* Forward-project device volume d_vol into device sinogram d_prj for the views
  * view_begin + k*view_stride, k < view_count; d_prj is [view_count][DNU][DNV].
  * The kernel of DD3_gpu_proj_branchless_sat2d_ker (iterativeRecon/DD3_GPU.cu:844-990).
@@ -115,6 +129,14 @@ int ctr_plan_fp(ctr_plan* plan, const float* d_vol, float* d_prj, int view_begin
  * CTR_BP_MATCHED (exact transpose of ctr_plan_fp).  Asynchronous. */
 int ctr_plan_bp(ctr_plan* plan, const float* d_prj, float* d_vol, int view_begin,
                 int view_stride, int view_count, int bp_kind, void* stream);
+
+/* ctr_plan_fp / ctr_plan_bp with BF16 storage (build addition): d_vol and d_prj of
+ * the forward projector, and d_prj of the back projector, are BF16 arrays; the back
+ * projector's output d_vol is FP32.  Asynchronous. */
+int ctr_plan_fp_bf16(ctr_plan* plan, const unsigned short* d_vol, unsigned short* d_prj,
+                     int view_begin, int view_stride, int view_count, void* stream);
+int ctr_plan_bp_bf16(ctr_plan* plan, const unsigned short* d_prj, float* d_vol, int view_begin,
+                     int view_stride, int view_count, int bp_kind, void* stream);
 
 /* OS-SART, iterativeRecon/os_sart.m:1-44, on the device.  d_x (in/out) is the
  * volume iterate, d_prj the full measured sinogram [PN][DNU][DNV].  Subset s holds
@@ -169,6 +191,12 @@ int ctr_memset(void* d_dst, int value, unsigned long long bytes);
 int ctr_synchronize(int device);
 /* Plain copy kernel of n floats (also the byte-counter calibration kernel). Asynchronous. */
 int ctr_device_copy(float* d_dst, const float* d_src, unsigned long long n, void* stream);
+
+/* n floats rounded to BF16 (nearest even) / n BF16 values widened to FP32. Asynchronous. */
+int ctr_f32_to_bf16(unsigned short* d_dst, const float* d_src, unsigned long long n,
+                    void* stream);
+int ctr_bf16_to_f32(float* d_dst, const unsigned short* d_src, unsigned long long n,
+                    void* stream);
 
 /* Message for the last failing call on this thread ("" if none). */
 const char* ctr_last_error(void);

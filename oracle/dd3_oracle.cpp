@@ -530,8 +530,9 @@ void back_any(const Geo<double>& g, const TI* prj, double* vol, int kind) {
 }
 
 // os_sart.m:13-44 in FP64 (relaxation 1, no clamp, guarded divisions).
+// trace (may be NULL) receives the iterate after every sweep: trace[it * NV + i].
 void os_sart(const Geo<double>& g, double* x, const float* b, int osNum, int iterNum, int kind,
-             double* resid_out) {
+             double* resid_out, double* trace) {
   const size_t NV = (size_t)g.XN * g.YN * g.ZN, NR = (size_t)g.DNU * g.DNV;
   std::vector<Geo<double>> sg;
   std::vector<std::vector<double>> rowSum(osNum), colSum(osNum);
@@ -569,12 +570,13 @@ void os_sart(const Geo<double>& g, double* x, const float* b, int osNum, int ite
       }
     }
     if (resid_out) resid_out[it] = std::sqrt(rr / bb);  // residual seen during the sweep
+    if (trace) std::copy(x, x + NV, trace + (size_t)it * NV);
   }
 }
 
 // conjgrad.m:4-27 in FP64 with A = Back(Proj(.)), b = Back(prj).
 void conjgrad(const Geo<double>& g, double* x, const float* prj, int iterNum, int kind,
-              double* resid_out) {
+              double* resid_out, double* trace) {
   const size_t NV = (size_t)g.XN * g.YN * g.ZN, NS = (size_t)g.DNU * g.DNV * g.PN;
   std::vector<double> sino(NS), b(NV), r0(NV), p0(NV), t(NV);
   auto A = [&](const double* in, double* out) {
@@ -600,6 +602,7 @@ void conjgrad(const Geo<double>& g, double* x, const float* prj, int iterNum, in
     const double beta = r1r1 / tmp1;
     for (size_t i = 0; i < NV; ++i) p0[i] = r0[i] + beta * p0[i];
     if (resid_out) resid_out[it] = std::sqrt(r1r1);
+    if (trace) std::copy(x, x + NV, trace + (size_t)it * NV);
   }
 }
 
@@ -673,15 +676,19 @@ void orc_bp_sat_f32(ORC_GEO_ARGS, float* vol, const float* prj, float dx, float 
 extern "C" {
 
 // x (in/out, FP64 volume) is updated in place; resid[iterNum] receives the
-// relative residual seen during each sweep.  kind: 0 reference bp, 1 matched.
+// relative residual seen during each sweep; trace (NULL, or iterNum volumes) every
+// iterate.  kind: 0 reference bp, 1 matched.
 void orc_os_sart_f64(ORC_GEO_ARGS, double* x, const float* prj, float dx, float dz,
-                     const uint8_t* mask, int osNum, int iterNum, int kind, double* resid) {
-  os_sart(make_geo<double>(ORC_GEO_PASS, dx, dz, mask), x, prj, osNum, iterNum, kind, resid);
+                     const uint8_t* mask, int osNum, int iterNum, int kind, double* resid,
+                     double* trace) {
+  os_sart(make_geo<double>(ORC_GEO_PASS, dx, dz, mask), x, prj, osNum, iterNum, kind, resid,
+          trace);
 }
 
 void orc_conjgrad_f64(ORC_GEO_ARGS, double* x, const float* prj, float dx, float dz,
-                      const uint8_t* mask, int iterNum, int kind, double* resid) {
-  conjgrad(make_geo<double>(ORC_GEO_PASS, dx, dz, mask), x, prj, iterNum, kind, resid);
+                      const uint8_t* mask, int iterNum, int kind, double* resid,
+                      double* trace) {
+  conjgrad(make_geo<double>(ORC_GEO_PASS, dx, dz, mask), x, prj, iterNum, kind, resid, trace);
 }
 
 // FP64 forward projection of an FP64 volume (direct form), for residuals of

@@ -128,28 +128,34 @@ def boundaries(c):
     return out
 
 
-def os_sart(g, b, x0=None, osNum=10, iterNum=1, bp_kind="ref"):
-    """FP64 OS-SART (os_sart.m:13-44); returns (x, per-sweep residuals)."""
+def os_sart(g, b, x0=None, osNum=10, iterNum=1, bp_kind="ref", trace=False):
+    """FP64 OS-SART (os_sart.m:13-44); returns (x, per-sweep residuals), plus the
+    iterate after every sweep ([iterNum, *vol_shape]) when trace is set."""
     x = np.zeros(g.vol_shape, np.float64) if x0 is None else np.array(x0, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float32)
     res = np.zeros(iterNum, dtype=np.float64)
+    tr = np.zeros((iterNum,) + g.vol_shape, np.float64) if trace else None
     args, keep = _geo_args(g)
     m = _mask(g)
     lib().orc_os_sart_f64(*args, _ptr(x), _ptr(b), _f(g.dx), _f(g.dz), _ptr(m), _i(osNum),
-                          _i(iterNum), _i(1 if bp_kind == "matched" else 0), _ptr(res))
-    return x, res
+                          _i(iterNum), _i(1 if bp_kind == "matched" else 0), _ptr(res),
+                          _ptr(tr) if trace else None)
+    return (x, res, tr) if trace else (x, res)
 
 
-def conjgrad(g, b, x0=None, iterNum=1, bp_kind="ref"):
-    """FP64 CG on Back(Proj(x)) = Back(b) (conjgrad.m:4-27); returns (x, ||r_k||)."""
+def conjgrad(g, b, x0=None, iterNum=1, bp_kind="ref", trace=False):
+    """FP64 CG on Back(Proj(x)) = Back(b) (conjgrad.m:4-27); returns (x, ||r_k||), plus
+    every iterate when trace is set."""
     x = np.zeros(g.vol_shape, np.float64) if x0 is None else np.array(x0, dtype=np.float64)
     b = np.ascontiguousarray(b, dtype=np.float32)
     res = np.zeros(iterNum, dtype=np.float64)
+    tr = np.zeros((iterNum,) + g.vol_shape, np.float64) if trace else None
     args, keep = _geo_args(g)
     m = _mask(g)
     lib().orc_conjgrad_f64(*args, _ptr(x), _ptr(b), _f(g.dx), _f(g.dz), _ptr(m), _i(iterNum),
-                           _i(1 if bp_kind == "matched" else 0), _ptr(res))
-    return x, res
+                           _i(1 if bp_kind == "matched" else 0), _ptr(res),
+                           _ptr(tr) if trace else None)
+    return (x, res, tr) if trace else (x, res)
 
 
 def num_threads():
