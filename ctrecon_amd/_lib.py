@@ -37,6 +37,9 @@ EXPORTS = [
     "ctr_residual", "ctr_device_count",
     "ctr_malloc", "ctr_free", "ctr_memcpy_h2d", "ctr_memcpy_d2h", "ctr_memset",
     "ctr_synchronize", "ctr_device_copy", "ctr_last_error", "ctr_version",
+    "ctr_unet_num_params", "ctr_unet_create", "ctr_unet_destroy", "ctr_unet_set_params",
+    "ctr_unet_get_params", "ctr_unet_set_scales", "ctr_unet_forward", "ctr_unet_loss_grad",
+    "ctr_unet_train_step",
 ]
 
 
@@ -104,12 +107,23 @@ def lib():
         L.ctr_memset.argtypes = [p, i, ctypes.c_ulonglong]
         L.ctr_synchronize.argtypes = [i]
         L.ctr_device_copy.argtypes = [p, p, ctypes.c_ulonglong, p]
+        ll = ctypes.c_longlong
+        L.ctr_unet_num_params.argtypes = []
+        L.ctr_unet_create.argtypes = [i, ctypes.POINTER(p)]
+        L.ctr_unet_destroy.argtypes = [p]
+        L.ctr_unet_set_params.argtypes = [p, p, ll]
+        L.ctr_unet_get_params.argtypes = [p, p, ll]
+        L.ctr_unet_set_scales.argtypes = [p, f, f, f]
+        L.ctr_unet_forward.argtypes = [p, p, p, p, i, i, i, p]
+        L.ctr_unet_loss_grad.argtypes = [p, p, p, p, i, i, i, p, ctypes.POINTER(d)]
+        L.ctr_unet_train_step.argtypes = [p, p, p, p, i, i, i, f, ctypes.POINTER(d)]
         L.ctr_last_error.restype = ctypes.c_char_p
         L.ctr_version.restype = ctypes.c_char_p
         for name in EXPORTS:
             fn = getattr(L, name)
             if name not in ("ctr_last_error", "ctr_version"):
                 fn.restype = ctypes.c_int
+        L.ctr_unet_num_params.restype = ctypes.c_longlong
         _lib = L
     return _lib
 

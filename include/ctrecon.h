@@ -181,6 +181,35 @@ int ctr_conjgrad(ctr_plan* plan, float* d_x, const float* d_prj, int iterNum, in
 /* Relative data residual ||A x - b|| / ||b|| over all views (FP64 sums). Synchronous. */
 int ctr_residual(ctr_plan* plan, const float* d_x, const float* d_prj, double* out);
 
+/* ---- learned warm start (build addition, no reference counterpart) ----
+ * A 3-level 3-D U-Net (16/32/64 channels, 3x3x3 convolutions, ReLU, 2x2x2 average
+ * pooling, nearest upsampling, concatenated skips, 1x1x1 head) run with BF16
+ * activations and weights and FP32 accumulation on the matrix cores; FP32 master
+ * parameters and Adam.  Input: the OS-SART iterate x and the last sweep's update
+ * delta (FP32 volumes, vol[(y*X + x)*Z + z]); output: the predicted update
+ * s_out * net(x * s_x, delta * s_d).  Grids: Y, X multiples of 4, Z a multiple of 32.
+ * Parameters: for each of the 10 convolutions W[27][Cout][Cin] then b[Cout], then the
+ * head's 16 weights and bias (ctr_unet_num_params() floats). */
+typedef struct ctr_unet ctr_unet;
+long long ctr_unet_num_params(void);
+int ctr_unet_create(int device, ctr_unet** out);
+int ctr_unet_destroy(ctr_unet* net);
+/* Host parameters in / out (set also resets the Adam moments). */
+int ctr_unet_set_params(ctr_unet* net, const float* host, long long n);
+int ctr_unet_get_params(ctr_unet* net, float* host, long long n);
+int ctr_unet_set_scales(ctr_unet* net, float s_x, float s_d, float s_out);
+/* d_update = predicted update (device FP32 volume).  Asynchronous on stream. */
+int ctr_unet_forward(ctr_unet* net, const float* d_x, const float* d_delta, float* d_update,
+                     int Y, int X, int Z, void* stream);
+/* Loss sum((net - d_target / s_out)^2) / (2 V) of the unscaled output and its gradient
+ * with respect to every parameter (host_grads, may be NULL).  Synchronous. */
+int ctr_unet_loss_grad(ctr_unet* net, const float* d_x, const float* d_delta,
+                       const float* d_target, int Y, int X, int Z, float* host_grads,
+                       double* loss);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on that loss. loss may be NULL. */
+int ctr_unet_train_step(ctr_unet* net, const float* d_x, const float* d_delta,
+                        const float* d_target, int Y, int X, int Z, float lr, double* loss);
+
 /* ---- device memory helpers (so that hosts need no other GPU runtime) ---- */
 int ctr_device_count(int* count);
 int ctr_malloc(void** d_ptr, unsigned long long bytes, int device);
