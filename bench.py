@@ -1,21 +1,28 @@
-"""Benchmark: time to relative data residual 1e-3 of helical OS-SART at 512x512x256.
+"""Benchmark: time to relative data residual 1e-3 of helical OS-SART at 512x512x256,
+learned warm start + BF16 storage vs plain FP32, on one GPU.
 
 One step is one OS-SART sweep (20 subsets; every view forward- and back-projected
 once) over the clinical-size helical scan: 512x512x256 volume at 0.7 mm, 888x64
 detector, 984 views/turn x 4 turns (3936 views), pitch 0.9, FOV mask r^2 < 0.99.
 The measured sinogram b = A x_true (FP32, noise-free) of a seeded 40-ellipsoid
-phantom is resident in HBM before timing starts; x_0 = 0.  The timed solve
-includes the row/column-sum set-up.  ``value`` is the wall time until the relative
-data residual ||b - A x|| / ||b|| (FP64 sums, accumulated during each sweep) first
-reaches the target.
+phantom is resident in HBM before timing starts; x_0 = 0.
+
+Two forms are timed on the same b, each from x_0 = 0 until the relative data residual
+||b - A x|| / ||b|| (FP64 sums, accumulated during each sweep) first reaches the
+target, with the row / column-sum set-up inside the timed region:
+  * plain FP32 OS-SART;
+  * learned + BF16: BF16 projector operands (FP32 master iterate), two sweeps, the
+    warm-start network's predicted update, then plain OS-SART sweeps (with the
+    fallback to plain OS-SART when the residual does not drop).
+``value`` is the learned + BF16 time when trained weights are present
+(ctrecon_amd/weights/warm_start.npz), otherwise the plain FP32 time.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 
-    python bench.py [--steps K] [--warmup W] [--target 1e-3] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--target 1e-3] [--form both|plain|learned] [--no-cpu]
 
 --steps 0 (default) runs sweeps until the target is met (at most --max-steps);
---steps K runs exactly K sweeps and reads the time-to-target off the per-sweep
-HIP-event times.
+--steps K runs exactly K sweeps of the plain form (profiling runs).
 """
 
 from __future__ import annotations
@@ -81,6 +88,9 @@ def main():
     ap.add_argument("--bp", default="ref", choices=["ref", "matched"])
     ap.add_argument("--cpu-views", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--form", default="both", choices=["both", "plain", "learned"])
+    ap.add_argument("--weights", default=os.path.join(ROOT, "ctrecon_amd", "weights",
+                                                      "warm_start.npz"))
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run the plain copy kernel (byte-counter calibration)")
     args = ap.parse_args()
@@ -119,71 +129,109 @@ def main():
     plan.fp(d_true, d_b)
     plan.synchronize()
     d_x = DeviceArray(g.vol_shape)
+    m = g.mask.reshape(g.YN, g.XN).astype(bool)
 
-    if args.warmup > 0:
-        d_x.zero()
-        plan.os_sart_run(d_x, d_b, args.subsets, args.warmup, args.bp)
-        log(f"warmup: {args.warmup} sweep(s) done")
+    def rmse_of(d):
+        x = d.get()
+        return float(np.sqrt(np.mean((x - x_true)[m] ** 2)))
 
-    d_x.zero()
-    plan.synchronize()
+    have_net = os.path.exists(args.weights)
+    if args.form == "learned" and not have_net:
+        log(f"no network weights at {args.weights}")
+        sys.exit(1)
+    run_learned = args.form in ("both", "learned") and have_net and args.steps == 0
+    run_plain = args.form in ("both", "plain") or not run_learned
     max_iter = args.steps if args.steps > 0 else args.max_steps
     stop = 0.0 if args.steps > 0 else args.target
-    t0 = time.perf_counter()
-    res, sweep_ms, st = plan.os_sart_run(d_x, d_b, args.subsets, max_iter, args.bp, stop)
-    plan.synchronize()
-    wall = time.perf_counter() - t0
-    steps = len(res)
-    cum = st["setup_ms"] + np.cumsum(sweep_ms)
-    hit = np.nonzero(res <= args.target)[0]
-    t_target = float(cum[hit[0]]) / 1e3 if hit.size else None
-    sweeps_target = int(hit[0]) + 1 if hit.size else None
-    log(f"{steps} sweeps, wall {wall:.2f}s, residual {res[-1]:.3e}, "
-        f"time to target {t_target}")
 
-    true_res = plan.residual(d_x, d_b)
-    x = d_x.get()
-    m = g.mask.reshape(g.YN, g.XN).astype(bool)
-    rmse = float(np.sqrt(np.mean((x - x_true)[m] ** 2)))
+    plain = st = None
+    if run_plain:
+        if args.warmup > 0:
+            d_x.zero()
+            plan.os_sart_run(d_x, d_b, args.subsets, args.warmup, args.bp)
+            log(f"warmup: {args.warmup} sweep(s) done")
+        d_x.zero()
+        plan.synchronize()
+        t0 = time.perf_counter()
+        res, sweep_ms, st = plan.os_sart_run(d_x, d_b, args.subsets, max_iter, args.bp, stop)
+        plan.synchronize()
+        wall = time.perf_counter() - t0
+        cum = st["setup_ms"] + np.cumsum(sweep_ms)
+        hit = np.nonzero(res <= args.target)[0]
+        plain = {
+            "time_to_target_s": float(cum[hit[0]]) / 1e3 if hit.size else None,
+            "wall_s": wall,
+            "sweeps": len(res),
+            "sweeps_to_target": int(hit[0]) + 1 if hit.size else None,
+            "setup_ms": st["setup_ms"],
+            "ms_per_sweep": float(np.mean(sweep_ms)) if len(res) else None,
+            "residual_sweep_final": float(res[-1]) if len(res) else None,
+            "residual_true_final": plan.residual(d_x, d_b),
+            "rmse_vs_truth_in_mask": rmse_of(d_x),
+        }
+        log(f"plain FP32: {len(res)} sweeps, wall {wall:.2f} s, residual {res[-1]:.3e}, "
+            f"time to target {plain['time_to_target_s']}")
 
-    # roofline of the dominant kernel (the back-projection update)
-    fp_avg = st["fp_ms"] / max(st["fp_launches"], 1)
-    bp_avg = st["bp_ms"] / max(st["bp_launches"], 1)
-    cone = plan.cone_voxels()
-    frame = g.DNU * g.DNV
-    bytes_per_view = 4.0 * cone + 4.0 * frame
-    per_launch_bytes = float(bytes_per_view.sum()) / args.subsets
-    views_per_launch = g.PN / args.subsets
-    dominant = "bp_ref_kernel" if args.bp == "ref" else "bp_matched_kernel"
-    dom_ms, other_ms = (bp_avg, fp_avg)
-    achieved = per_launch_bytes / (dom_ms / 1e3)
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get(dominant)
-    out = {
-        "metric": METRIC,
-        "value": t_target,
-        "unit": "s",
-        "n_gpus": 1,
-        "steps": steps,
-        "warmup": args.warmup,
-        "ms_per_step": float(np.mean(sweep_ms)) if steps else None,
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic: seeded 40-ellipsoid phantom (seed 1) + uniform texture; "
-                "b = A x_true (noise-free FP32)",
-        "config": {
-            "workload": "512x512x256 helical OS-SART, plain FP32, reference back projector"
-            if args.bp == "ref" else "512x512x256 helical OS-SART, plain FP32, matched adjoint",
-            "form": "plain FP32 (the learned + BF16 form is not built yet)",
-            "volume": [512, 512, 256], "voxel_mm": 0.7, "detector": [888, 64],
-            "views": int(g.PN), "views_per_turn": 984, "pitch": 0.9, "subsets": args.subsets,
-            "target_rel_residual": args.target,
-        },
-        "roofline": {
+    learned = None
+    if run_learned:
+        from ctrecon_amd.learned import warm_start_os_sart
+        from ctrecon_amd.unet import UNet
+
+        net = UNet.from_file(args.weights)
+        plan.set_storage("bf16")
+        # warm-up: one BF16 sweep and one network pass (code loading, network buffers)
+        d_x.zero()
+        plan.os_sart_run(d_x, d_b, args.subsets, 1, args.bp)
+        tmp = DeviceArray(g.vol_shape)
+        net.forward(d_x, d_x, tmp)
+        plan.synchronize()
+        del tmp
+        d_x.zero()
+        plan.synchronize()
+        t0 = time.perf_counter()
+        info = warm_start_os_sart(plan, net, d_b, d_x, args.subsets, args.target, max_iter,
+                                  args.bp)
+        wall = time.perf_counter() - t0
+        plan.set_storage("f32")
+        r = np.array(info["residuals"])
+        learned = {
+            "time_to_target_s": wall if r[-1] <= args.target else None,
+            "wall_s": wall,
+            "sweeps": int(info["sweeps"]),
+            "fallback_taken": bool(info["fallback"]),
+            "phase_s": {k: info[k] for k in ("setup_s", "pre_sweeps_s", "network_s",
+                                             "post_sweeps_s")},
+            "residual_sweep_final": float(r[-1]),
+            "residual_true_final": plan.residual(d_x, d_b),
+            "rmse_vs_truth_in_mask": rmse_of(d_x),
+            "weights": os.path.relpath(args.weights, ROOT),
+        }
+        log(f"learned + BF16: {info['sweeps']} sweeps, wall {wall:.2f} s, residual {r[-1]:.3e}, "
+            f"fallback {info['fallback']}")
+
+    main_form = learned if learned is not None else plain
+    t_target = main_form["time_to_target_s"]
+    steps = main_form["sweeps"]
+    sweeps_target = (learned["sweeps"] if learned is not None and t_target is not None
+                     else plain["sweeps_to_target"])
+
+    roof = kernels = pps = None
+    if st is not None:
+        # roofline of the dominant kernel (the back-projection update)
+        fp_avg = st["fp_ms"] / max(st["fp_launches"], 1)
+        bp_avg = st["bp_ms"] / max(st["bp_launches"], 1)
+        cone = plan.cone_voxels()
+        frame = g.DNU * g.DNV
+        bytes_per_view = 4.0 * cone + 4.0 * frame
+        per_launch_bytes = float(bytes_per_view.sum()) / args.subsets
+        views_per_launch = g.PN / args.subsets
+        dominant = "bp_ref_kernel" if args.bp == "ref" else "bp_matched_kernel"
+        achieved = per_launch_bytes / (bp_avg / 1e3)
+        traffic = None
+        tpath = os.path.join(ROOT, "profiles", "traffic.json")
+        if os.path.exists(tpath):
+            traffic = json.load(open(tpath)).get(dominant)
+        roof = {
             "kernel": dominant,
             "bound": "hbm",
             "achieved": achieved / 1e9,
@@ -192,23 +240,55 @@ def main():
             "frac": achieved / HBM_PEAK,
             "traffic": traffic,
             "algorithmic_bytes_per_launch": per_launch_bytes,
-            "avg_launch_ms": dom_ms,
-        },
-        "sweeps_to_target": sweeps_target,
-        "setup_ms": st["setup_ms"],
-        "residual_sweep_final": float(res[-1]) if steps else None,
-        "residual_true_final": true_res,
-        "rmse_vs_truth_in_mask": rmse,
-        "projections_per_s": {
-            "os_sart_loop": g.PN / (np.mean(sweep_ms) / 1e3) if steps else None,
+            "avg_launch_ms": bp_avg,
+        }
+        kernels = {"fp_fused": fp_avg, "bp_update": bp_avg}
+        pps = {
+            "os_sart_loop": g.PN / (plain["ms_per_sweep"] / 1e3) if plain["sweeps"] else None,
             "fp_kernel": views_per_launch / (fp_avg / 1e3),
             "bp_kernel": views_per_launch / (bp_avg / 1e3),
+        }
+    form = ("learned warm start + BF16 storage (FP32 accumulation)" if learned is not None else
+            "plain FP32" + ("" if have_net else " (no trained network weights found)"))
+    out = {
+        "metric": METRIC,
+        "value": t_target,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": (learned["phase_s"]["post_sweeps_s"] * 1e3 / max(learned["sweeps"] - 2, 1)
+                        if learned is not None else plain["ms_per_sweep"]),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 accumulation, bf16 storage" if learned is not None else "f32",
+        "data": "synthetic: seeded 40-ellipsoid phantom (seed 1) + uniform texture; "
+                "b = A x_true (noise-free FP32)",
+        "config": {
+            "workload": "512x512x256 helical OS-SART to rel. residual "
+                        f"{args.target:g}, " + ("reference back projector" if args.bp == "ref"
+                                                 else "matched adjoint"),
+            "form": form,
+            "volume": [512, 512, 256], "voxel_mm": 0.7, "detector": [888, 64],
+            "views": int(g.PN), "views_per_turn": 984, "pitch": 0.9, "subsets": args.subsets,
+            "target_rel_residual": args.target,
         },
-        "kernel_avg_ms": {"fp_fused": fp_avg, "bp_update": bp_avg},
+        "roofline": roof,
+        "sweeps_to_target": sweeps_target,
+        "plain_fp32": plain,
+        "learned_bf16": learned,
+        "speedup_vs_plain": (plain["wall_s"] / learned["wall_s"]
+                             if plain is not None and learned is not None else None),
+        "rmse_ratio_vs_plain": (learned["rmse_vs_truth_in_mask"] / plain["rmse_vs_truth_in_mask"]
+                                if plain is not None and learned is not None else None),
+        "projections_per_s": pps,
+        "kernel_avg_ms": kernels,
     }
     if not args.no_cpu:
         log("cpu baseline sample")
-        out["cpu_baseline"] = cpu_baseline(g, x_true, args.cpu_views, sweeps_target)
+        out["cpu_baseline"] = cpu_baseline(g, x_true, args.cpu_views,
+                                           plain["sweeps_to_target"] if plain else sweeps_target)
     print(json.dumps(out))
 
 

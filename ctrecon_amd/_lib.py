@@ -36,7 +36,8 @@ EXPORTS = [
     "ctr_plan_bp", "ctr_os_sart", "ctr_os_sart_run", "ctr_cone_voxels", "ctr_conjgrad",
     "ctr_residual", "ctr_device_count",
     "ctr_malloc", "ctr_free", "ctr_memcpy_h2d", "ctr_memcpy_d2h", "ctr_memset",
-    "ctr_synchronize", "ctr_device_copy", "ctr_last_error", "ctr_version",
+    "ctr_synchronize", "ctr_device_copy", "ctr_device_sub", "ctr_device_axpy", "ctr_last_error", "ctr_version",
+    "ctr_sart_create", "ctr_sart_sweeps", "ctr_sart_setup_ms", "ctr_sart_destroy",
     "ctr_unet_num_params", "ctr_unet_create", "ctr_unet_destroy", "ctr_unet_set_params",
     "ctr_unet_get_params", "ctr_unet_set_scales", "ctr_unet_forward", "ctr_unet_loss_grad",
     "ctr_unet_train_step",
@@ -107,6 +108,12 @@ def lib():
         L.ctr_memset.argtypes = [p, i, ctypes.c_ulonglong]
         L.ctr_synchronize.argtypes = [i]
         L.ctr_device_copy.argtypes = [p, p, ctypes.c_ulonglong, p]
+        L.ctr_device_sub.argtypes = [p, p, p, ctypes.c_ulonglong, p]
+        L.ctr_device_axpy.argtypes = [p, p, d, ctypes.c_ulonglong, p]
+        L.ctr_sart_create.argtypes = [p, p, i, i, ctypes.POINTER(p)]
+        L.ctr_sart_sweeps.argtypes = [p, p, i, d, p, p, ctypes.POINTER(CtrSartStats)]
+        L.ctr_sart_setup_ms.argtypes = [p]
+        L.ctr_sart_destroy.argtypes = [p]
         ll = ctypes.c_longlong
         L.ctr_unet_num_params.argtypes = []
         L.ctr_unet_create.argtypes = [i, ctypes.POINTER(p)]
@@ -124,6 +131,7 @@ def lib():
             if name not in ("ctr_last_error", "ctr_version"):
                 fn.restype = ctypes.c_int
         L.ctr_unet_num_params.restype = ctypes.c_longlong
+        L.ctr_sart_setup_ms.restype = ctypes.c_double
         _lib = L
     return _lib
 

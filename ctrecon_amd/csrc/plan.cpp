@@ -391,6 +391,18 @@ struct ctr_plan {
   Plan* impl;
 };
 
+// An OS-SART session: the subset row / column sums of one measured sinogram, kept so
+// that sweeps can be run in several calls (os_sart.m:14-18 computes them once).
+struct ctr_sart {
+  Plan* P = nullptr;
+  const float* prj = nullptr;
+  int osNum = 1, kind = CTR_BP_REFERENCE, maxV = 0;
+  bool bf = false;
+  double setup_ms = 0.0;
+  DBuf rowInv, colInv, tp, xs, sums;
+  std::vector<size_t> rowOff;
+};
+
 namespace {
 
 Plan* get(ctr_plan* p) { return p ? p->impl : nullptr; }
@@ -585,79 +597,130 @@ int ctr_os_sart(ctr_plan* plan, float* d_x, const float* d_prj, int osNum, int i
 int ctr_os_sart_run(ctr_plan* plan, float* d_x, const float* d_prj, int osNum, int maxIter,
                     int bp_kind, double stop_resid, double* resid, double* sweep_ms,
                     ctr_sart_stats* stats) {
+  ctr_sart* S = nullptr;
+  int rc = ctr_sart_create(plan, d_prj, osNum, bp_kind, &S);
+  if (rc) return rc;
+  rc = ctr_sart_sweeps(S, d_x, maxIter, stop_resid, resid, sweep_ms, stats);
+  ctr_sart_destroy(S);
+  return rc;
+}
+
+int ctr_sart_create(ctr_plan* plan, const float* d_prj, int osNum, int bp_kind, ctr_sart** out) {
+  if (!out) return fail(CTR_ERR_ARG, "out is NULL");
+  *out = nullptr;
   Plan* P = get(plan);
-  if (!P || !d_x || !d_prj) return fail(CTR_ERR_ARG, "NULL argument");
-  if (osNum < 1 || osNum > P->d.PN || maxIter < 0)
-    return fail(CTR_ERR_ARG, "osNum must be in [1, PN] and iterNum >= 0");
+  if (!P || !d_prj) return fail(CTR_ERR_ARG, "NULL argument");
+  if (osNum < 1 || osNum > P->d.PN) return fail(CTR_ERR_ARG, "osNum must be in [1, PN]");
   if (bp_kind != CTR_BP_REFERENCE && bp_kind != CTR_BP_MATCHED)
     return fail(CTR_ERR_ARG, "bad bp_kind");
   CTR_HIP(hipSetDevice(P->device));
+  std::unique_ptr<ctr_sart> S(new ctr_sart());
+  S->P = P;
+  S->prj = d_prj;
+  S->osNum = osNum;
+  S->kind = bp_kind;
+  S->bf = P->storage == CTR_STORAGE_BF16;
   const Dims& d = P->d;
   const size_t NV = (size_t)d.XN * d.YN * d.ZN, frame = (size_t)d.DNU * d.DNV;
-  const int maxV = (d.PN + osNum - 1) / osNum;
+  S->maxV = (d.PN + osNum - 1) / osNum;
   int rc;
-  DBuf rowInv, colInv, tp, ones, sums;
-  if ((rc = rowInv.alloc(frame * d.PN * sizeof(float)))) return rc;  // subset-compact order
-  if ((rc = colInv.alloc(NV * osNum * sizeof(float)))) return rc;
-  // BF16 storage: the forward projector reads the shadow xs of the FP32 iterate and
-  // writes the subset residual as BF16; the back projector updates x and xs together
-  const bool bf = P->storage == CTR_STORAGE_BF16;
-  DBuf xs;
-  if ((rc = tp.alloc(frame * maxV * (bf ? sizeof(uint16_t) : sizeof(float))))) return rc;
-  if (bf && (rc = xs.alloc(NV * sizeof(uint16_t)))) return rc;
-  if ((rc = ones.alloc(std::max(NV, frame * maxV) * sizeof(float)))) return rc;
-  if ((rc = sums.alloc(2 * sizeof(double) * osNum))) return rc;
-  if ((rc = ensure_partials(*P, 2 * (size_t)fp_partial_blocks(*P, maxV) + 2))) return rc;
+  if ((rc = S->rowInv.alloc(frame * d.PN * sizeof(float)))) return rc;  // subset-compact order
+  if ((rc = S->colInv.alloc(NV * osNum * sizeof(float)))) return rc;
+  if ((rc = S->tp.alloc(frame * S->maxV * (S->bf ? sizeof(uint16_t) : sizeof(float))))) return rc;
+  if (S->bf && (rc = S->xs.alloc(NV * sizeof(uint16_t)))) return rc;
+  if ((rc = S->sums.alloc(2 * sizeof(double) * osNum))) return rc;
+  if ((rc = ensure_partials(*P, 2 * (size_t)fp_partial_blocks(*P, S->maxV) + 2))) return rc;
   hipStream_t st = nullptr;
-  const bool timing = stats != nullptr || sweep_ms != nullptr;
-  // events: [0] setup start, [1] setup end / sweep 0 start, then one per sweep end;
-  // per-launch pairs for the kernel totals
   EventPool ev;
-  if (timing && (rc = ev.init(2 + maxIter + 4 * osNum))) return rc;
-  if (stats) std::memset(stats, 0, sizeof(*stats));
-  if (timing) CTR_HIP(hipEventRecord(ev.e[0], st));
+  if ((rc = ev.init(2))) return rc;
+  CTR_HIP(hipEventRecord(ev.e[0], st));
   // RowSum{s} = fp(ones), ColSum{s} = bp(ones) (os_sart.m:14-18)
-  if ((rc = launch_fill(ones.f(), std::max(NV, frame * maxV), 1.0f, st))) return rc;
-  std::vector<size_t> rowOff(osNum);
+  DBuf ones;
+  if ((rc = ones.alloc(std::max(NV, frame * S->maxV) * sizeof(float)))) return rc;
+  if ((rc = launch_fill(ones.f(), std::max(NV, frame * S->maxV), 1.0f, st))) return rc;
+  S->rowOff.resize(osNum);
   size_t off = 0;
   for (int s = 0; s < osNum; ++s) {
     const int vc = (d.PN - s + osNum - 1) / osNum;
-    rowOff[s] = off;
-    float* rI = rowInv.f() + off * frame;
-    if ((rc = fp_any(*P, ones.f(), rI, s, osNum, vc, st, nullptr, nullptr, nullptr))) return rc;
+    S->rowOff[s] = off;
+    float* rI = S->rowInv.f() + off * frame;
+    if ((rc = fp_any<float>(*P, ones.f(), rI, s, osNum, vc, st, nullptr, nullptr, nullptr)))
+      return rc;
     if ((rc = launch_inv_guard(rI, rI, frame * vc, st))) return rc;
-    float* cI = colInv.f() + NV * s;
-    if ((rc = bp_any(*P, ones.f(), cI, s, osNum, vc, bp_kind, st, nullptr, 0))) return rc;
+    float* cI = S->colInv.f() + NV * s;
+    if ((rc = bp_any<float>(*P, ones.f(), cI, s, osNum, vc, bp_kind, st, nullptr, 0))) return rc;
     if ((rc = launch_inv_guard(cI, cI, NV, st))) return rc;
     off += vc;
   }
-  if (bf && (rc = launch_to_bf16(xs.h(), d_x, NV, st))) return rc;
-  if (timing) CTR_HIP(hipEventRecord(ev.e[1], st));
-  hipEvent_t* kev = timing ? ev.e.data() + 2 + maxIter : nullptr;
+  CTR_HIP(hipEventRecord(ev.e[1], st));
+  CTR_HIP(hipEventSynchronize(ev.e[1]));
+  float ms = 0;
+  CTR_HIP(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
+  S->setup_ms = ms;
+  *out = S.release();
+  return CTR_OK;
+}
+
+int ctr_sart_destroy(ctr_sart* S) {
+  if (S) {
+    (void)hipSetDevice(S->P->device);
+    delete S;
+  }
+  return CTR_OK;
+}
+
+double ctr_sart_setup_ms(const ctr_sart* S) { return S ? S->setup_ms : 0.0; }
+
+int ctr_sart_sweeps(ctr_sart* S, float* d_x, int maxIter, double stop_resid, double* resid,
+                    double* sweep_ms, ctr_sart_stats* stats) {
+  if (!S || !d_x) return fail(CTR_ERR_ARG, "NULL argument");
+  if (maxIter < 0) return fail(CTR_ERR_ARG, "iterNum must be >= 0");
+  Plan* P = S->P;
+  CTR_HIP(hipSetDevice(P->device));
+  const Dims& d = P->d;
+  const int osNum = S->osNum;
+  const size_t NV = (size_t)d.XN * d.YN * d.ZN, frame = (size_t)d.DNU * d.DNV;
+  int rc;
+  hipStream_t st = nullptr;
+  const bool timing = stats != nullptr || sweep_ms != nullptr;
+  // events: [0] start, then one per sweep end; per-launch pairs for the kernel totals
+  EventPool ev;
+  if (timing && (rc = ev.init(1 + maxIter + 4 * osNum))) return rc;
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->setup_ms = S->setup_ms;
+  }
+  if (timing) CTR_HIP(hipEventRecord(ev.e[0], st));
+  // the caller may have changed x since the last call: refresh its BF16 shadow
+  if (S->bf && (rc = launch_to_bf16(S->xs.h(), d_x, NV, st))) return rc;
+  hipEvent_t* kev = timing ? ev.e.data() + 1 + maxIter : nullptr;
   int done = 0;
   for (int it = 0; it < maxIter; ++it) {
     for (int s = 0; s < osNum; ++s) {
       const int vc = (d.PN - s + osNum - 1) / osNum;
       if (stats) CTR_HIP(hipEventRecord(kev[4 * s + 0], st));
       // tp = (prj - A_s x) ./ RowSum (os_sart.m:24,30-31)
-      const float* rI = rowInv.f() + rowOff[s] * frame;
-      rc = bf ? fp_any<uint16_t>(*P, xs.h(), tp.h(), s, osNum, vc, st, d_prj, rI, P->partials)
-              : fp_any<float>(*P, d_x, tp.f(), s, osNum, vc, st, d_prj, rI, P->partials);
+      const float* rI = S->rowInv.f() + S->rowOff[s] * frame;
+      rc = S->bf ? fp_any<uint16_t>(*P, S->xs.h(), S->tp.h(), s, osNum, vc, st, S->prj, rI,
+                                    P->partials)
+                 : fp_any<float>(*P, d_x, S->tp.f(), s, osNum, vc, st, S->prj, rI, P->partials);
       if (rc) return rc;
       if (stats) CTR_HIP(hipEventRecord(kev[4 * s + 1], st));
-      if ((rc = launch_reduce_pairs(P->partials, fp_partial_blocks(*P, vc), sums.d() + 2 * s, st)))
+      if ((rc = launch_reduce_pairs(P->partials, fp_partial_blocks(*P, vc), S->sums.d() + 2 * s,
+                                    st)))
         return rc;
       if (stats) CTR_HIP(hipEventRecord(kev[4 * s + 2], st));
       // x += B_s(tp) ./ ColSum (os_sart.m:32,37-40)
-      const float* cI = colInv.f() + NV * s;
-      rc = bf ? bp_any<uint16_t>(*P, tp.h(), d_x, s, osNum, vc, bp_kind, st, cI, 1, xs.h())
-              : bp_any<float>(*P, tp.f(), d_x, s, osNum, vc, bp_kind, st, cI, 1);
+      const float* cI = S->colInv.f() + NV * s;
+      rc = S->bf ? bp_any<uint16_t>(*P, S->tp.h(), d_x, s, osNum, vc, S->kind, st, cI, 1,
+                                    S->xs.h())
+                 : bp_any<float>(*P, S->tp.f(), d_x, s, osNum, vc, S->kind, st, cI, 1);
       if (rc) return rc;
       if (stats) CTR_HIP(hipEventRecord(kev[4 * s + 3], st));
     }
-    if (timing) CTR_HIP(hipEventRecord(ev.e[2 + it], st));
+    if (timing) CTR_HIP(hipEventRecord(ev.e[1 + it], st));
     std::vector<double> h(2 * osNum);
-    CTR_HIP(hipMemcpy(h.data(), sums.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    CTR_HIP(hipMemcpy(h.data(), S->sums.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
     double rr = 0, bb = 0;
     for (int s = 0; s < osNum; ++s) {
       rr += h[2 * s];
@@ -678,19 +741,14 @@ int ctr_os_sart_run(ctr_plan* plan, float* d_x, const float* d_prj, int osNum, i
     }
     if (sweep_ms) {
       float ms = 0;
-      CTR_HIP(hipEventElapsedTime(&ms, ev.e[1 + it], ev.e[2 + it]));
+      CTR_HIP(hipEventElapsedTime(&ms, ev.e[it], ev.e[1 + it]));
       sweep_ms[it] = ms;
     }
     done = it + 1;
     if (stop_resid > 0 && r <= stop_resid) break;
   }
   CTR_HIP(hipDeviceSynchronize());
-  if (stats) {
-    float ms = 0;
-    CTR_HIP(hipEventElapsedTime(&ms, ev.e[0], ev.e[1]));
-    stats->setup_ms = ms;
-    stats->sweeps = done;
-  }
+  if (stats) stats->sweeps = done;
   return CTR_OK;
 }
 
@@ -888,6 +946,17 @@ int ctr_memset(void* d_dst, int value, unsigned long long bytes) {
 int ctr_device_copy(float* d_dst, const float* d_src, unsigned long long n, void* stream) {
   if (!d_dst || !d_src) return fail(CTR_ERR_ARG, "NULL pointer");
   return launch_copy(d_dst, d_src, n, (hipStream_t)stream);
+}
+
+int ctr_device_sub(float* d_out, const float* d_a, const float* d_b, unsigned long long n,
+                   void* stream) {
+  if (!d_out || !d_a || !d_b) return fail(CTR_ERR_ARG, "NULL pointer");
+  return launch_sub(d_out, d_a, d_b, n, (hipStream_t)stream);
+}
+
+int ctr_device_axpy(float* d_y, const float* d_x, double a, unsigned long long n, void* stream) {
+  if (!d_y || !d_x) return fail(CTR_ERR_ARG, "NULL pointer");
+  return launch_axpy(d_y, d_x, a, n, (hipStream_t)stream);
 }
 
 int ctr_synchronize(int device) {

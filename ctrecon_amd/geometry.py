@@ -169,6 +169,16 @@ def config_clinical():
                           voxel_z=0.7, col_height_at_iso=0.625)
 
 
+def config_training(N_xy=256, N_z=128):
+    """The learned warm start's training scans: the clinical detector and trajectory
+    (888x64, 984 views/turn, 4 turns, pitch 0.9) over a smaller grid at the same
+    0.7 mm voxels (default 256x256x128: the central 179 mm x 90 mm)."""
+    return helical_config(sid=541, sdd=949, DNU=888, DNV=64, col_width=1.0239,
+                          col_height=1.096439, views_per_turn=984, turns=4,
+                          feed_mm_per_turn=0.9 * 64 * 0.625, N_xy=N_xy, N_z=N_z, voxel_xy=0.7,
+                          voxel_z=0.7, col_height_at_iso=0.625)
+
+
 def fov_mask(n):
     """FOV mask of iterativeRecon/myDemo.m:40-48: ((i-0.5-N/2)/(N/2))^2 + ((j-0.5-N/2)/(N/2))^2
     < 0.99 for one-based i, j.  Returned as uint8 [n, n] (symmetric)."""

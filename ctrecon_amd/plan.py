@@ -16,7 +16,7 @@ from . import _lib
 from ._lib import CTR_BP_MATCHED, CTR_BP_REFERENCE, CtrGeometry, CtrSartStats, check, lib
 from .geometry import Geometry
 
-__all__ = ["DeviceArray", "Plan", "device_count", "bp_kind_code", "to_bf16", "from_bf16"]
+__all__ = ["DeviceArray", "Plan", "Sart", "device_count", "bp_kind_code", "to_bf16", "from_bf16"]
 
 
 def device_count() -> int:
@@ -62,6 +62,11 @@ class DeviceArray:
         check(lib().ctr_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes))
         return out
 
+    def copy_from(self, other: "DeviceArray"):
+        """Device-to-device copy of a float32 array of the same size."""
+        assert other.nbytes == self.nbytes and self.dtype == np.float32
+        check(lib().ctr_device_copy(self.ptr, other.ptr, self.nbytes // 4, None))
+
     def zero(self):
         check(lib().ctr_memset(self.ptr, 0, self.nbytes))
 
@@ -73,6 +78,41 @@ class DeviceArray:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class Sart:
+    """An OS-SART session (``ctr_sart_*``): row / column sums computed once, then sweeps
+    in any number of calls on an iterate the caller may change in between."""
+
+    def __init__(self, plan: "Plan", d_prj: DeviceArray, osNum: int, kind="ref"):
+        self.plan = plan
+        self.prj = d_prj  # keeps the sinogram alive
+        h = ctypes.c_void_p()
+        check(lib().ctr_sart_create(plan.handle, d_prj.ptr, osNum, bp_kind_code(kind),
+                                    ctypes.byref(h)))
+        self.handle = h.value
+        self.setup_ms = float(lib().ctr_sart_setup_ms(self.handle))
+
+    def sweeps(self, d_x: DeviceArray, n: int, stop_resid=0.0):
+        """Runs up to n sweeps; returns (residuals, sweep_ms, stats) of those run."""
+        res = np.zeros(max(n, 1), dtype=np.float64)
+        ms = np.zeros(max(n, 1), dtype=np.float64)
+        st = CtrSartStats()
+        check(lib().ctr_sart_sweeps(self.handle, d_x.ptr, n, float(stop_resid), res.ctypes.data,
+                                    ms.ctypes.data, ctypes.byref(st)))
+        k = st.sweeps
+        return res[:k], ms[:k], {f: getattr(st, f) for f, _ in CtrSartStats._fields_}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().ctr_sart_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 

@@ -165,6 +165,21 @@ int ctr_os_sart_run(ctr_plan* plan, float* d_x, const float* d_prj, int osNum, i
                     int bp_kind, double stop_resid, double* resid, double* sweep_ms,
                     ctr_sart_stats* stats);
 
+/* OS-SART session (build addition): the subset row / column sums of one measured
+ * sinogram (os_sart.m:14-18) computed once by ctr_sart_create, then any number of
+ * ctr_sart_sweeps calls on an iterate that the caller may modify in between (the
+ * learned warm start does).  The session uses the plan's method and storage at
+ * creation; d_prj must stay valid until ctr_sart_destroy.  Sessions of one plan must
+ * not run concurrently (they share the plan's reduction scratch). */
+typedef struct ctr_sart ctr_sart;
+int ctr_sart_create(ctr_plan* plan, const float* d_prj, int osNum, int bp_kind,
+                    ctr_sart** out);
+/* Sweeps as in ctr_os_sart_run (stats->setup_ms is the session's set-up time). */
+int ctr_sart_sweeps(ctr_sart* sart, float* d_x, int maxIter, double stop_resid, double* resid,
+                    double* sweep_ms, ctr_sart_stats* stats);
+double ctr_sart_setup_ms(const ctr_sart* sart);
+int ctr_sart_destroy(ctr_sart* sart);
+
 /* Number of masked voxels inside each view's cone (the voxels a view can see:
  * z-cells meeting [sz + (bz_0 - z0) l/S2D, sz + (bz_DNV - z0) l/S2D] at in-plane
  * source distance l), for views view_begin + k*view_stride.  Host computation;
@@ -226,6 +241,12 @@ int ctr_f32_to_bf16(unsigned short* d_dst, const float* d_src, unsigned long lon
                     void* stream);
 int ctr_bf16_to_f32(float* d_dst, const unsigned short* d_src, unsigned long long n,
                     void* stream);
+
+/* d_out = d_a - d_b and d_y += a * d_x over n floats (the solvers' vector steps, for
+ * hosts that keep iterates on the device). Asynchronous. */
+int ctr_device_sub(float* d_out, const float* d_a, const float* d_b, unsigned long long n,
+                   void* stream);
+int ctr_device_axpy(float* d_y, const float* d_x, double a, unsigned long long n, void* stream);
 
 /* Message for the last failing call on this thread ("" if none). */
 const char* ctr_last_error(void);
