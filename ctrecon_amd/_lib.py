@@ -38,6 +38,8 @@ EXPORTS = [
     "ctr_malloc", "ctr_free", "ctr_memcpy_h2d", "ctr_memcpy_d2h", "ctr_memset",
     "ctr_synchronize", "ctr_device_copy", "ctr_device_sub", "ctr_device_axpy", "ctr_last_error", "ctr_version",
     "ctr_sart_create", "ctr_sart_sweeps", "ctr_sart_setup_ms", "ctr_sart_destroy",
+    "ctr_batch_create", "ctr_batch_sweeps", "ctr_batch_set_graph", "ctr_batch_set_tuning",
+    "ctr_batch_get_info", "ctr_batch_destroy",
     "ctr_unet_num_params", "ctr_unet_create", "ctr_unet_destroy", "ctr_unet_set_params",
     "ctr_unet_get_params", "ctr_unet_set_scales", "ctr_unet_forward", "ctr_unet_loss_grad",
     "ctr_unet_train_step",
@@ -68,6 +70,15 @@ class CtrSartStats(ctypes.Structure):
     _fields_ = [
         ("sweeps", ctypes.c_int), ("fp_launches", ctypes.c_int), ("bp_launches", ctypes.c_int),
         ("setup_ms", ctypes.c_double), ("fp_ms", ctypes.c_double), ("bp_ms", ctypes.c_double),
+    ]
+
+
+class CtrBatchInfo(ctypes.Structure):
+    _fields_ = [
+        ("fp_colw", ctypes.c_int), ("fp_grid", ctypes.c_int), ("bp_zg", ctypes.c_int),
+        ("bp_grid", ctypes.c_int), ("setup_ms", ctypes.c_double), ("tune_ms", ctypes.c_double),
+        ("capture_ms", ctypes.c_double), ("fp_items", ctypes.c_longlong),
+        ("bp_items", ctypes.c_longlong),
     ]
 
 
@@ -114,6 +125,12 @@ def lib():
         L.ctr_sart_sweeps.argtypes = [p, p, i, d, p, p, ctypes.POINTER(CtrSartStats)]
         L.ctr_sart_setup_ms.argtypes = [p]
         L.ctr_sart_destroy.argtypes = [p]
+        L.ctr_batch_create.argtypes = [p, i, p, p, i, i, ctypes.POINTER(p)]
+        L.ctr_batch_sweeps.argtypes = [p, i, d, p, p, ctypes.POINTER(i)]
+        L.ctr_batch_set_graph.argtypes = [p, i]
+        L.ctr_batch_set_tuning.argtypes = [p, i, i, i, i]
+        L.ctr_batch_get_info.argtypes = [p, ctypes.POINTER(CtrBatchInfo)]
+        L.ctr_batch_destroy.argtypes = [p]
         ll = ctypes.c_longlong
         L.ctr_unet_num_params.argtypes = []
         L.ctr_unet_create.argtypes = [i, ctypes.POINTER(p)]

@@ -1,0 +1,468 @@
+// Batched OS-SART over several helical scans of uneven size (the C ABI ctr_batch_*).
+//
+// One sweep is osNum subset steps; in step s every scan forward-projects its subset s
+// (fused residual), the residual partials of every scan are reduced, and every scan
+// back-projects and updates its iterate.  Each of the three is ONE launch over all
+// scans: persistent workgroups walk a flat work list whose per-scan ranges and
+// operands come from descriptor tables built here once (device memory, one table per
+// subset step).  A whole sweep (3 * osNum launches) is captured into a HIP graph and
+// replayed; the residual readback between sweeps stays outside the graph.  The
+// persistent grid sizes, the forward work mapping and the back projector's z groups
+// per lane are chosen by timing the candidates on the batch itself (autotune).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ctr_internal.hpp"
+
+using namespace ctr;
+
+namespace {
+
+struct Mem {
+  void* p = nullptr;
+  ~Mem() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t bytes) {
+    if (hipMalloc(&p, std::max<size_t>(bytes, 1)) != hipSuccess) {
+      p = nullptr;
+      return fail(CTR_ERR_NOMEM, "batch: device allocation failed");
+    }
+    return CTR_OK;
+  }
+  float* f() const { return (float*)p; }
+  double* d() const { return (double*)p; }
+  uint16_t* h() const { return (uint16_t*)p; }
+};
+
+struct Scan {
+  Plan* P = nullptr;
+  float* x = nullptr;
+  const float* prj = nullptr;
+  size_t NV = 0, frame = 0;
+  Mem rowInv, colInv, tp, xs, partials;
+  std::vector<size_t> rowOff;
+};
+
+int vcount(const Plan& P, int s, int osNum) { return (P.d.PN - s + osNum - 1) / osNum; }
+
+}  // namespace
+
+struct ctr_batch {
+  int n = 0, osNum = 1, kind = CTR_BP_REFERENCE, device = 0, ncu = 256;
+  bool bf = false;
+  std::vector<std::unique_ptr<Scan>> scans;
+  Mem sums, zeros;  // sums: [n][osNum][2]; zeros: the largest volume (tuning)
+  Mem fpItems, bpItems, tuneBpItems, redItems;
+  std::vector<int> fpTotal, bpTotal;
+  bool colw = false;
+  int fpGrid = 0, zg = 1, bpGrid = 0;
+  hipStream_t st = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  bool useGraph = true;
+  double setup_ms = 0, tune_ms = 0, capture_ms = 0;
+  ~ctr_batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
+namespace {
+
+// Descriptor tables for the current tuning (zg and colw change the work lists).
+template <typename T>
+int build_tables(ctr_batch& B) {
+  const int n = B.n, S = B.osNum;
+  std::vector<FpBatchItem<T>> fp((size_t)S * n);
+  std::vector<BpBatchItem<T>> bp((size_t)S * n), tbp((size_t)S * n);
+  std::vector<ReduceItem> red((size_t)S * n);
+  B.fpTotal.assign(S, 0);
+  B.bpTotal.assign(S, 0);
+  for (int s = 0; s < S; ++s) {
+    int fb = 0, bb = 0;
+    for (int i = 0; i < n; ++i) {
+      Scan& c = *B.scans[i];
+      const Plan& P = *c.P;
+      const int vc = vcount(P, s, S);
+      const T* vol = B.bf ? (const T*)c.xs.p : (const T*)c.x;
+      T* tp = (T*)c.tp.p;
+      FpBatchItem<T>& f = fp[(size_t)s * n + i];
+      f.a = fp_args<T>(P, vol, tp, s, S, c.prj, c.rowInv.f() + c.rowOff[s] * c.frame,
+                       c.partials.d());
+      f.begin = fb;
+      f.upv = fp_blocks_per_view(P, B.colw);
+      fb += f.upv * std::max(vc, 0);
+      red[(size_t)s * n + i] = ReduceItem{c.partials.d(), f.upv * std::max(vc, 0),
+                                          B.sums.d() + ((size_t)i * S + s) * 2};
+      BpBatchItem<T>& b = bp[(size_t)s * n + i];
+      b.a = bp_args<T>(P, tp, c.x, s, S, std::max(vc, 0), c.colInv.f() + c.NV * s,
+                       B.bf ? c.xs.h() : nullptr);
+      b.begin = bb;
+      b.colBlocks = (P.d.XN * P.d.YN + 3) / 4;
+      const int zblocks = (P.d.ZN + 64 * B.zg - 1) / (64 * B.zg);
+      bb += vc > 0 ? b.colBlocks * zblocks : 0;
+      BpBatchItem<T>& t = tbp[(size_t)s * n + i];
+      t = b;
+      t.a.invCol = B.zeros.f();  // x += acc * 0: timing runs leave the iterate unchanged
+    }
+    B.fpTotal[s] = fb;
+    B.bpTotal[s] = bb;
+  }
+  auto up = [&](Mem& m, const void* src, size_t bytes) -> int {
+    if (!m.p) {
+      int rc = m.alloc(bytes);
+      if (rc) return rc;
+    }
+    CTR_HIP(hipMemcpy(m.p, src, bytes, hipMemcpyHostToDevice));
+    return CTR_OK;
+  };
+  int rc;
+  if ((rc = up(B.fpItems, fp.data(), fp.size() * sizeof(fp[0])))) return rc;
+  if ((rc = up(B.bpItems, bp.data(), bp.size() * sizeof(bp[0])))) return rc;
+  if ((rc = up(B.tuneBpItems, tbp.data(), tbp.size() * sizeof(tbp[0])))) return rc;
+  return up(B.redItems, red.data(), red.size() * sizeof(red[0]));
+}
+
+int build(ctr_batch& B) { return B.bf ? build_tables<uint16_t>(B) : build_tables<float>(B); }
+
+template <typename T>
+int step_fp(ctr_batch& B, int s) {
+  return launch_fp_batch<T>((const FpBatchItem<T>*)B.fpItems.p + (size_t)s * B.n, B.n,
+                            B.fpTotal[s], B.colw, B.fpGrid, B.st);
+}
+
+template <typename T>
+int step_bp(ctr_batch& B, int s, bool tuning) {
+  const Mem& m = tuning ? B.tuneBpItems : B.bpItems;
+  return launch_bp_batch<T>((const BpBatchItem<T>*)m.p + (size_t)s * B.n, B.n, B.bpTotal[s],
+                            B.zg, B.bpGrid, B.st);
+}
+
+int sweep_launches(ctr_batch& B) {
+  for (int s = 0; s < B.osNum; ++s) {
+    int rc = B.bf ? step_fp<uint16_t>(B, s) : step_fp<float>(B, s);
+    if (rc) return rc;
+    if ((rc = launch_reduce_batch((const ReduceItem*)B.redItems.p + (size_t)s * B.n, B.n, B.st)))
+      return rc;
+    if ((rc = B.bf ? step_bp<uint16_t>(B, s, false) : step_bp<float>(B, s, false))) return rc;
+  }
+  return CTR_OK;
+}
+
+// Mean duration (ms) of reps launches of f after one warm-up launch.
+template <typename F>
+int time_launches(ctr_batch& B, F f, int reps, float* ms) {
+  hipEvent_t a, b;
+  CTR_HIP(hipEventCreate(&a));
+  CTR_HIP(hipEventCreate(&b));
+  int rc = f();
+  if (!rc) {
+    (void)hipEventRecord(a, B.st);
+    for (int r = 0; r < reps && !rc; ++r) rc = f();
+    (void)hipEventRecord(b, B.st);
+    (void)hipEventSynchronize(b);
+    float t = 0;
+    (void)hipEventElapsedTime(&t, a, b);
+    *ms = t / reps;
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return rc;
+}
+
+void drop_graph(ctr_batch& B) {
+  if (B.exec) (void)hipGraphExecDestroy(B.exec);
+  if (B.graph) (void)hipGraphDestroy(B.graph);
+  B.exec = nullptr;
+  B.graph = nullptr;
+}
+
+int autotune(ctr_batch& B) {
+  bool all64 = true;
+  int maxZN = 0;
+  for (auto& c : B.scans) {
+    all64 = all64 && c->P->d.DNV == 64;
+    maxZN = std::max(maxZN, c->P->d.ZN);
+  }
+  const int mult[3] = {2, 4, 8};
+  float best = 1e30f;
+  bool bcolw = false;
+  int bgrid = 0, rc;
+  for (int cw = all64 ? 1 : 0; cw >= 0; --cw) {
+    B.colw = cw;
+    if ((rc = build(B))) return rc;
+    for (int m : mult) {
+      B.fpGrid = m * B.ncu;
+      float ms = 0;
+      auto f = [&]() { return B.bf ? step_fp<uint16_t>(B, 0) : step_fp<float>(B, 0); };
+      if ((rc = time_launches(B, f, 2, &ms))) return rc;
+      if (ms < best) {
+        best = ms;
+        bcolw = cw;
+        bgrid = B.fpGrid;
+      }
+    }
+  }
+  B.colw = bcolw;
+  B.fpGrid = bgrid;
+  best = 1e30f;
+  int bzg = 1;
+  bgrid = 0;
+  const int zgs[3] = {1, 2, 4};
+  for (int zg : zgs) {
+    if (zg > 1 && 64 * (zg / 2) >= maxZN) break;  // more groups than any column needs
+    B.zg = zg;
+    if ((rc = build(B))) return rc;
+    for (int m : mult) {
+      B.bpGrid = m * B.ncu;
+      float ms = 0;
+      auto f = [&]() {
+        return B.bf ? step_bp<uint16_t>(B, 0, true) : step_bp<float>(B, 0, true);
+      };
+      if ((rc = time_launches(B, f, 2, &ms))) return rc;
+      if (ms < best) {
+        best = ms;
+        bzg = zg;
+        bgrid = B.bpGrid;
+      }
+    }
+  }
+  B.zg = bzg;
+  B.bpGrid = bgrid;
+  return build(B);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ctr_batch_create(ctr_plan* const* plans, int nscan, float* const* d_x,
+                     const float* const* d_prj, int osNum, int bp_kind, ctr_batch** out) {
+  if (!out) return fail(CTR_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (!plans || !d_x || !d_prj || nscan < 1) return fail(CTR_ERR_ARG, "empty batch");
+  if (bp_kind != CTR_BP_REFERENCE)
+    return fail(CTR_ERR_UNSUPPORTED, "batched OS-SART runs the reference back projector");
+  std::unique_ptr<ctr_batch> B(new ctr_batch());
+  B->n = nscan;
+  B->osNum = osNum;
+  B->kind = bp_kind;
+  for (int i = 0; i < nscan; ++i) {
+    if (!plans[i] || !plans[i]->impl || !d_x[i] || !d_prj[i])
+      return fail(CTR_ERR_ARG, "NULL plan or buffer in the batch");
+    Plan* P = plans[i]->impl;
+    if (i == 0) {
+      B->device = P->device;
+      B->bf = P->storage == CTR_STORAGE_BF16;
+    } else if (P->device != B->device || (P->storage == CTR_STORAGE_BF16) != B->bf) {
+      return fail(CTR_ERR_ARG, "the plans of a batch must share device and storage");
+    }
+    if (P->method != CTR_METHOD_DIRECT)
+      return fail(CTR_ERR_UNSUPPORTED, "batched OS-SART uses the direct projector method");
+    if (osNum < 1 || osNum > P->d.PN) return fail(CTR_ERR_ARG, "osNum must be in [1, PN]");
+  }
+  CTR_HIP(hipSetDevice(B->device));
+  (void)hipDeviceGetAttribute(&B->ncu, hipDeviceAttributeMultiprocessorCount, B->device);
+  CTR_HIP(hipStreamCreate(&B->st));
+  int rc;
+  hipEvent_t e0, e1;
+  CTR_HIP(hipEventCreate(&e0));
+  CTR_HIP(hipEventCreate(&e1));
+  (void)hipEventRecord(e0, B->st);
+  size_t maxNV = 0;
+  for (int i = 0; i < nscan; ++i) {
+    auto c = std::make_unique<Scan>();
+    Plan& P = *plans[i]->impl;
+    c->P = &P;
+    c->x = d_x[i];
+    c->prj = d_prj[i];
+    c->NV = (size_t)P.d.XN * P.d.YN * P.d.ZN;
+    c->frame = (size_t)P.d.DNU * P.d.DNV;
+    maxNV = std::max(maxNV, c->NV);
+    const int maxV = vcount(P, 0, osNum);
+    const int upv = std::max(fp_blocks_per_view(P, true), fp_blocks_per_view(P, false));
+    if ((rc = c->rowInv.alloc(c->frame * P.d.PN * sizeof(float)))) return rc;
+    if ((rc = c->colInv.alloc(c->NV * osNum * sizeof(float)))) return rc;
+    if ((rc = c->tp.alloc(c->frame * maxV * (B->bf ? 2 : 4)))) return rc;
+    if (B->bf && (rc = c->xs.alloc(c->NV * 2))) return rc;
+    if ((rc = c->partials.alloc((size_t)2 * upv * maxV * sizeof(double)))) return rc;
+    // row / column sums of the scan's subsets (os_sart.m:14-18)
+    Mem ones;
+    const size_t no = std::max(c->NV, c->frame * maxV);
+    if ((rc = ones.alloc(no * sizeof(float)))) return rc;
+    if ((rc = launch_fill(ones.f(), no, 1.0f, B->st))) return rc;
+    c->rowOff.resize(osNum);
+    size_t off = 0;
+    for (int s = 0; s < osNum; ++s) {
+      const int vc = vcount(P, s, osNum);
+      c->rowOff[s] = off;
+      float* rI = c->rowInv.f() + off * c->frame;
+      if ((rc = launch_fp<float>(P, ones.f(), rI, s, osNum, vc, B->st, nullptr, nullptr,
+                                 nullptr)))
+        return rc;
+      if ((rc = launch_inv_guard(rI, rI, c->frame * vc, B->st))) return rc;
+      float* cI = c->colInv.f() + c->NV * s;
+      if ((rc = launch_bp<float>(P, ones.f(), cI, s, osNum, vc, CTR_BP_REFERENCE, B->st, nullptr,
+                                 0, nullptr)))
+        return rc;
+      if ((rc = launch_inv_guard(cI, cI, c->NV, B->st))) return rc;
+      off += vc;
+    }
+    CTR_HIP(hipStreamSynchronize(B->st));  // before `ones` is freed
+    B->scans.push_back(std::move(c));
+  }
+  (void)hipEventRecord(e1, B->st);
+  CTR_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  B->setup_ms = ms;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if ((rc = B->sums.alloc((size_t)nscan * osNum * 2 * sizeof(double)))) return rc;
+  if ((rc = B->zeros.alloc(maxNV * sizeof(float)))) return rc;
+  CTR_HIP(hipMemsetAsync(B->zeros.p, 0, maxNV * sizeof(float), B->st));
+  if (B->bf)
+    for (auto& c : B->scans)
+      if ((rc = launch_to_bf16(c->xs.h(), c->x, c->NV, B->st))) return rc;
+  const auto t0 = std::chrono::steady_clock::now();
+  if ((rc = autotune(*B))) return rc;
+  CTR_HIP(hipStreamSynchronize(B->st));
+  B->tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  *out = B.release();
+  return CTR_OK;
+}
+
+int ctr_batch_set_graph(ctr_batch* B, int enable) {
+  if (!B) return fail(CTR_ERR_ARG, "NULL batch");
+  B->useGraph = enable != 0;
+  return CTR_OK;
+}
+
+int ctr_batch_set_tuning(ctr_batch* B, int fp_colw, int fp_grid, int bp_zg, int bp_grid) {
+  if (!B) return fail(CTR_ERR_ARG, "NULL batch");
+  if (bp_zg < 1 || bp_zg > 4 || fp_grid < 1 || bp_grid < 1)
+    return fail(CTR_ERR_ARG, "bad tuning");
+  if (fp_colw)
+    for (auto& c : B->scans)
+      if (c->P->d.DNV != 64) return fail(CTR_ERR_ARG, "column-per-wave mapping needs DNV == 64");
+  CTR_HIP(hipSetDevice(B->device));
+  B->colw = fp_colw != 0;
+  B->fpGrid = fp_grid;
+  B->zg = bp_zg;
+  B->bpGrid = bp_grid;
+  drop_graph(*B);
+  return build(*B);
+}
+
+int ctr_batch_get_info(const ctr_batch* B, ctr_batch_info* info) {
+  if (!B || !info) return fail(CTR_ERR_ARG, "NULL argument");
+  info->fp_colw = B->colw;
+  info->fp_grid = B->fpGrid;
+  info->bp_zg = B->zg;
+  info->bp_grid = B->bpGrid;
+  info->setup_ms = B->setup_ms;
+  info->tune_ms = B->tune_ms;
+  info->capture_ms = B->capture_ms;
+  long long f = 0, b = 0;
+  for (int s = 0; s < B->osNum; ++s) {
+    f += B->fpTotal[s];
+    b += B->bpTotal[s];
+  }
+  info->fp_items = f;
+  info->bp_items = b;
+  return CTR_OK;
+}
+
+int ctr_batch_sweeps(ctr_batch* B, int maxIter, double stop_resid, double* resid,
+                     double* sweep_ms, int* done) {
+  if (!B) return fail(CTR_ERR_ARG, "NULL batch");
+  if (maxIter < 0) return fail(CTR_ERR_ARG, "maxIter must be >= 0");
+  CTR_HIP(hipSetDevice(B->device));
+  int rc;
+  // the caller may have changed the iterates: refresh the BF16 shadows
+  if (B->bf)
+    for (auto& c : B->scans)
+      if ((rc = launch_to_bf16(c->xs.h(), c->x, c->NV, B->st))) return rc;
+  if (B->useGraph && !B->exec) {
+    const auto t0 = std::chrono::steady_clock::now();
+    CTR_HIP(hipStreamBeginCapture(B->st, hipStreamCaptureModeThreadLocal));
+    rc = sweep_launches(*B);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(B->st, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(CTR_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    B->graph = g;
+    CTR_HIP(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+    B->capture_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  hipEvent_t e0, e1;
+  CTR_HIP(hipEventCreate(&e0));
+  CTR_HIP(hipEventCreate(&e1));
+  std::vector<double> h((size_t)B->n * B->osNum * 2);
+  int it = 0;
+  rc = CTR_OK;
+  for (; it < maxIter; ++it) {
+    (void)hipEventRecord(e0, B->st);
+    if (B->useGraph) {
+      if (hipGraphLaunch(B->exec, B->st) != hipSuccess) {
+        rc = fail(CTR_ERR_HIP, "hipGraphLaunch failed");
+        break;
+      }
+    } else if ((rc = sweep_launches(*B))) {
+      break;
+    }
+    (void)hipEventRecord(e1, B->st);
+    if (hipMemcpyAsync(h.data(), B->sums.p, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                       B->st) != hipSuccess ||
+        hipStreamSynchronize(B->st) != hipSuccess) {
+      rc = fail(CTR_ERR_HIP, "residual readback failed");
+      break;
+    }
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (sweep_ms) sweep_ms[it] = ms;
+    bool all = true;
+    for (int i = 0; i < B->n; ++i) {
+      double rr = 0, bb = 0;
+      for (int s = 0; s < B->osNum; ++s) {
+        rr += h[((size_t)i * B->osNum + s) * 2];
+        bb += h[((size_t)i * B->osNum + s) * 2 + 1];
+      }
+      const double r = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+      if (resid) resid[(size_t)it * B->n + i] = r;
+      all = all && r <= stop_resid;
+    }
+    if (stop_resid > 0 && all) {
+      ++it;
+      break;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (done) *done = it;
+  return rc;
+}
+
+int ctr_batch_destroy(ctr_batch* B) {
+  if (B) {
+    (void)hipSetDevice(B->device);
+    (void)hipStreamSynchronize(B->st);
+    delete B;
+  }
+  return CTR_OK;
+}
+
+}  // extern "C"

@@ -59,6 +59,65 @@ struct HostGeo {
   double ocx, ocy, ocz, dx, dz, S2D, bz0, bzN;
 };
 
+// Operands of one forward-projection launch over one scan (T: storage type of vol and
+// out, float or uint16_t holding BF16).  b / invRow / partials are set for the fused
+// OS-SART residual and NULL otherwise.
+template <typename T>
+struct FpArgs {
+  const T* vol;
+  T* out;
+  const ViewP* views;
+  const float2* bnd;
+  const float4* cols;
+  const float* Bz;
+  const float* h2;
+  const float* invdB;
+  const float* maskf;
+  const float2* mext;
+  Dims d;
+  int vb, vs;
+  const float* b;
+  const float* invRow;
+  double* partials;
+};
+
+// Operands of one reference back-projection launch over one scan (T: storage type of
+// prj; vol FP32).  invCol / shadow are set for the OS-SART update and NULL otherwise.
+template <typename T>
+struct BpArgs {
+  const T* prj;
+  float* vol;
+  const ViewP* views;
+  const double2* src64;
+  const uint8_t* mask;
+  Dims d;
+  BpConsts c;
+  BpConsts64 c64;
+  int vb, vs, vc;
+  const float* invCol;
+  uint16_t* shadow;
+};
+
+// Batched launches: one work list over several scans.  Work items [begin, begin + n)
+// of the flat list belong to the scan; upv = blocks per view (forward) and
+// colBlocks = blocks of 4 voxel columns per z block (back).
+template <typename T>
+struct FpBatchItem {
+  FpArgs<T> a;
+  int begin, upv;
+};
+template <typename T>
+struct BpBatchItem {
+  BpArgs<T> a;
+  int begin, colBlocks;
+};
+// Per-scan reduction of the fused residual partials of one batched subset step.
+struct ReduceItem {
+  const double* partials;
+  int n;
+  double* out;
+};
+
 struct Plan {
   int device = 0;
   Dims d{};
@@ -116,6 +175,21 @@ int launch_bp(const Plan& P, const T* prj, float* vol, int vb, int vs, int vc, i
 template <typename T>
 int launch_scale_matched(const Plan& P, const T* in, float* out, int vb, int vs, int vc,
                          hipStream_t st);
+template <typename T>
+FpArgs<T> fp_args(const Plan& P, const T* vol, T* out, int vb, int vs, const float* b,
+                  const float* invRow, double* partials);
+template <typename T>
+BpArgs<T> bp_args(const Plan& P, const T* prj, float* vol, int vb, int vs, int vc,
+                  const float* invCol, uint16_t* shadow);
+int fp_blocks_per_view(const Plan& P, bool colw);
+// batched persistent launches (grid = number of workgroups)
+template <typename T>
+int launch_fp_batch(const FpBatchItem<T>* items, int nscan, int total, bool colw, int grid,
+                    hipStream_t st);
+template <typename T>
+int launch_bp_batch(const BpBatchItem<T>* items, int nscan, int total, int zg, int grid,
+                    hipStream_t st);
+int launch_reduce_batch(const ReduceItem* items, int nscan, hipStream_t st);
 int launch_to_bf16(uint16_t* dst, const float* src, size_t n, hipStream_t st);
 int launch_from_bf16(float* dst, const uint16_t* src, size_t n, hipStream_t st);
 int launch_fill(float* p, size_t n, float v, hipStream_t st);
@@ -141,3 +215,8 @@ int launch_bp_sat(const Plan& P, const T* prj, float* sat, float* vol, int vb, i
                   hipStream_t st, const float* invCol, int accumulate, uint16_t* shadow);
 
 }  // namespace ctr
+
+// The opaque handle of the C ABI.
+struct ctr_plan {
+  ctr::Plan* impl;
+};

@@ -21,8 +21,8 @@
 namespace ctr {
 namespace {
 
-// Block-wide sum of two doubles into partials[2*block + {0,1}].
-__device__ __forceinline__ void block_sum2(double a, double b, double* partials, int block) {
+// Block-wide sum of two doubles into slot[0], slot[1].
+__device__ __forceinline__ void block_sum2(double a, double b, double* slot) {
   __shared__ double red[2][kWaves];
   for (int off = 32; off > 0; off >>= 1) {
     a += __shfl_xor(a, off, 64);
@@ -40,9 +40,10 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* partials,
       sa += red[0][i];
       sb += red[1][i];
     }
-    partials[2 * block] = sa;
-    partials[2 * block + 1] = sb;
+    slot[0] = sa;
+    slot[1] = sb;
   }
+  __syncthreads();  // red[] is reused by the next work item of a persistent block
 }
 
 // ---------------------------------------------------------------------------
@@ -58,33 +59,23 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* partials,
 // and the ray sum is accumulated in FP32.
 // ---------------------------------------------------------------------------
 template <bool FUSED, bool COLW, typename T>
-__global__ __launch_bounds__(kBlock) void fp_kernel(
-    const T* __restrict__ vol, T* __restrict__ out, const ViewP* __restrict__ views,
-    const float2* __restrict__ bnd, const float4* __restrict__ cols,
-    const float* __restrict__ Bz, const float* __restrict__ h2,
-    const float* __restrict__ invdB, const float* __restrict__ maskf,
-    const float2* __restrict__ mext, Dims d, int vb, int vs,
-    const float* __restrict__ b, const float* __restrict__ invRow,
-    double* __restrict__ partials) {
-  const int hw = blockIdx.y * gridDim.x + blockIdx.x;
-  const int lb = xcd_logical_block(hw, gridDim.x * gridDim.y);
-  const int k = lb / gridDim.x;
-  const int p = vb + k * vs;
+__device__ __forceinline__ void fp_body(const FpArgs<T>& A, int k, int ublk, double* slot) {
+  const Dims& d = A.d;
+  const int p = A.vb + k * A.vs;
   const int nray = d.DNU * d.DNV;
   // COLW (DNV == 64): a wave is exactly one detector column, so everything in-plane
   // is uniform and lives in scalar registers
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int ray = COLW ? ((lb - k * gridDim.x) * kWaves + wid) * 64 + (threadIdx.x & 63)
-                       : (lb - k * gridDim.x) * kBlock + threadIdx.x;
+  const int ray = COLW ? (ublk * kWaves + wid) * 64 + (threadIdx.x & 63) : ublk * kBlock + threadIdx.x;
   double res = 0.0, bsq = 0.0;
   if (ray < nray) {
-    const int u = COLW ? ((lb - k * gridDim.x) * kWaves + wid) : ray / d.DNV;
+    const int u = COLW ? (ublk * kWaves + wid) : ray / d.DNV;
     const int v = ray - u * d.DNV;
-    const ViewP vp = views[p];
-    const float2 bl = bnd[(size_t)p * (d.DNU + 1) + u];
-    const float2 br = bnd[(size_t)p * (d.DNU + 1) + u + 1];
-    const float4 ct = cols[(size_t)p * d.DNU + u];
-    const float Bv = Bz[v], Bv1 = Bz[v + 1];
+    const ViewP vp = A.views[p];
+    const float2 bl = A.bnd[(size_t)p * (d.DNU + 1) + u];
+    const float2 br = A.bnd[(size_t)p * (d.DNU + 1) + u + 1];
+    const float4 ct = A.cols[(size_t)p * d.DNU + u];
+    const float Bv = A.Bz[v], Bv1 = A.Bz[v + 1];
     int NS, NI, sS, cS, mS, mC;
     if (vp.xmarch) {
       NS = d.XN; NI = d.YN; sS = d.ZN; cS = d.XN * d.ZN; mS = 1; mC = d.XN;
@@ -93,8 +84,9 @@ __global__ __launch_bounds__(kBlock) void fp_kernel(
     }
     const float ZNh = (float)d.ZN - 0.5f;
     const int ZN1 = d.ZN - 1;
-    const float2* me = mext + (vp.xmarch ? 0 : d.XN);
-    const Buf<T> V(vol, (uint32_t)(d.XN * d.YN * d.ZN));
+    const float2* me = A.mext + (vp.xmarch ? 0 : d.XN);
+    const Buf<T> V(A.vol, (uint32_t)(d.XN * d.YN * d.ZN));
+    const float* maskf = A.maskf;
     float acc = 0.f;
     for (int ii = 0; ii < NS; ++ii) {
       const float fi = (float)ii;
@@ -152,22 +144,50 @@ __global__ __launch_bounds__(kBlock) void fp_kernel(
       }
       acc = fmaf(s, inv, acc);
     }
-    const float K = ct.w * __builtin_amdgcn_sqrtf(ct.z + h2[v]) * invdB[v];
+    const float K = ct.w * __builtin_amdgcn_sqrtf(ct.z + A.h2[v]) * A.invdB[v];
     const float val = acc * K;
     const size_t o = ((size_t)k * d.DNU + u) * d.DNV + v;
     if constexpr (FUSED) {
       const size_t og = ((size_t)p * d.DNU + u) * d.DNV + v;
-      const float bb = b[og];
+      const float bb = A.b[og];
       const float diff = bb - val;
-      st(out, o, invRow ? diff * invRow[o] : diff);
+      st(A.out, o, A.invRow ? diff * A.invRow[o] : diff);
       res = (double)diff * (double)diff;
       bsq = (double)bb * (double)bb;
     } else {
-      st(out, o, val);
+      st(A.out, o, val);
     }
   }
-  if constexpr (FUSED) block_sum2(res, bsq, partials, hw);
+  if constexpr (FUSED) block_sum2(res, bsq, slot);
 }
+
+template <bool FUSED, bool COLW, typename T>
+__global__ __launch_bounds__(kBlock) void fp_kernel(FpArgs<T> A) {
+  const int hw = blockIdx.y * gridDim.x + blockIdx.x;
+  const int lb = xcd_logical_block(hw, gridDim.x * gridDim.y);
+  const int k = lb / gridDim.x;
+  fp_body<FUSED, COLW, T>(A, k, lb - k * gridDim.x, A.partials + 2 * hw);
+}
+
+// Batched persistent forward projection: each workgroup takes one contiguous range of
+// the flat work list (XCD-aware, as in fp_kernel), items of one scan being
+// consecutive (view-major, then column blocks).
+template <bool FUSED, bool COLW, typename T>
+__global__ __launch_bounds__(kBlock) void fp_batch_kernel(const FpBatchItem<T>* __restrict__ D,
+                                                          int nscan, int total) {
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int per = (total + gridDim.x - 1) / gridDim.x;
+  const int i0 = lb * per, i1 = min(total, i0 + per);
+  int s = 0;
+  for (int item = i0; item < i1; ++item) {
+    while (s + 1 < nscan && D[s + 1].begin <= item) ++s;
+    const FpBatchItem<T>& B = D[s];
+    const int local = item - B.begin;
+    const int k = local / B.upv;
+    fp_body<FUSED, COLW, T>(B.a, k, local - k * B.upv, B.a.partials + 2 * local);
+  }
+}
+
 
 // ---------------------------------------------------------------------------
 // Reference back projector: one wave per (x, y) column, each lane owning ZG
@@ -183,19 +203,17 @@ __global__ __launch_bounds__(kBlock) void fp_kernel(
 // T: storage type of prj (float, or uint16_t for BF16); vol is FP32.
 // ---------------------------------------------------------------------------
 template <bool UPDATE, int ZG, typename T>
-__global__ __launch_bounds__(kBlock) void bp_ref_kernel(
-    const T* __restrict__ prj, float* __restrict__ vol, const ViewP* __restrict__ views,
-    const double2* __restrict__ src64, const uint8_t* __restrict__ mask, Dims d, BpConsts c,
-    BpConsts64 c64, int vb, int vs, int vc, const float* __restrict__ invCol,
-    uint16_t* __restrict__ shadow) {
+__device__ __forceinline__ void bp_ref_body(const BpArgs<T>& A, int col, int zblk) {
+  const Dims& d = A.d;
+  const BpConsts& c = A.c;
+  const BpConsts64& c64 = A.c64;
   const int lane = threadIdx.x & 63;
-  const int col = blockIdx.x * kWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if (col >= d.XN * d.YN) return;
-  const int zb0 = blockIdx.y * (64 * ZG);  // first z of the wave (uniform)
+  const int zb0 = zblk * (64 * ZG);  // first z of the wave (uniform)
   const int zb = zb0 + lane;
   const int y = col / d.XN, x = col - y * d.XN;
-  float* vcol = vol + (size_t)col * d.ZN;
-  if (mask[col] != 1) {
+  float* vcol = A.vol + (size_t)col * d.ZN;
+  if (A.mask[col] != 1) {
     if constexpr (!UPDATE) {
 #pragma unroll
       for (int g = 0; g < ZG; ++g)
@@ -218,9 +236,9 @@ __global__ __launch_bounds__(kBlock) void bp_ref_kernel(
   const float DNVh = (float)d.DNV - 0.5f;
   const int DNV = d.DNV;
   const uint32_t frame = (uint32_t)(d.DNU * d.DNV);
-  for (int k = 0; k < vc; ++k) {
-    const int p = vb + k * vs;
-    const ViewP vp = views[p];
+  for (int k = 0; k < A.vc; ++k) {
+    const int p = A.vb + k * A.vs;
+    const ViewP vp = A.views[p];
     // row shadow, FP32 (DD3_GPU.cu:1446-1450): t = (dZ +- hz) * S / l + cV; first the
     // whole column's row range (uniform), to skip views it does not meet
     const float dXf = Pxf - vp.sx, dYf = Pyf - vp.sy;
@@ -233,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void bp_ref_kernel(
       if (!(fmaxf(ta, tb) > -0.5f && fminf(ta, tb) < DNVh)) continue;
     }
     // fan-angle shadow, FP64 (DD3_GPU.cu:1447-1464); sy*dX - sx*dY == sy*Px - sx*Py
-    const double2 s64 = src64[p];
+    const double2 s64 = A.src64[p];
     const double dX = Px - s64.x, dY = Py - s64.y;
     const double l2 = dX * dX + dY * dY;
     const double rl = rsqrt(l2);
@@ -255,7 +273,7 @@ __global__ __launch_bounds__(kBlock) void bp_ref_kernel(
     // sign of the reference's corner difference: (tR - tL) and (tU - tD)
     const float sgn = ((tR < tL) != rneg) ? -1.f : 1.f;
     const float cUp = fmaf(c.hz, rlS, c.cV), cDp = fmaf(-c.hz, rlS, c.cV);
-    const Buf<T> pp(prj + (size_t)k * frame, frame);
+    const Buf<T> pp(A.prj + (size_t)k * frame, frame);
 #pragma unroll
     for (int g = 0; g < ZG; ++g) {
       const float dZ = fmaf((float)g, dzg, Pz0) - vp.sz;
@@ -307,14 +325,40 @@ __global__ __launch_bounds__(kBlock) void bp_ref_kernel(
     const int z = zb + 64 * g;
     if (z >= d.ZN) continue;
     if constexpr (UPDATE) {
-      const float nx = fmaf(acc[g], invCol[(size_t)col * d.ZN + z], vcol[z]);
+      const float nx = fmaf(acc[g], A.invCol[(size_t)col * d.ZN + z], vcol[z]);
       vcol[z] = nx;
-      if (shadow) shadow[(size_t)col * d.ZN + z] = to_bf16(nx);
+      if (A.shadow) A.shadow[(size_t)col * d.ZN + z] = to_bf16(nx);
     } else {
       vcol[z] = acc[g];
     }
   }
 }
+
+template <bool UPDATE, int ZG, typename T>
+__global__ __launch_bounds__(kBlock) void bp_ref_kernel(BpArgs<T> A) {
+  const int col = blockIdx.x * kWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  bp_ref_body<UPDATE, ZG, T>(A, col, blockIdx.y);
+}
+
+// Batched persistent reference back projection: flat work list of (scan, z block,
+// block of 4 columns), contiguous ranges per workgroup as in fp_batch_kernel.
+template <bool UPDATE, int ZG, typename T>
+__global__ __launch_bounds__(kBlock) void bp_batch_kernel(const BpBatchItem<T>* __restrict__ D,
+                                                          int nscan, int total) {
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int per = (total + gridDim.x - 1) / gridDim.x;
+  const int i0 = lb * per, i1 = min(total, i0 + per);
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int s = 0;
+  for (int item = i0; item < i1; ++item) {
+    while (s + 1 < nscan && D[s + 1].begin <= item) ++s;
+    const BpBatchItem<T>& B = D[s];
+    const int local = item - B.begin;
+    const int zblk = local / B.colBlocks;
+    bp_ref_body<UPDATE, ZG, T>(B.a, (local - zblk * B.colBlocks) * kWaves + wid, zblk);
+  }
+}
+
 
 // ---------------------------------------------------------------------------
 // Matched adjoint: the exact transpose of fp_kernel, in gather form.
@@ -530,7 +574,7 @@ __global__ __launch_bounds__(kBlock) void dot_partial_kernel(const float* __rest
   double s = 0.0;
   for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
     s += (double)a[i] * (double)b[i];
-  block_sum2(s, 0.0, partials, blockIdx.x);
+  block_sum2(s, 0.0, partials + 2 * blockIdx.x);
 }
 
 __global__ __launch_bounds__(kBlock) void reduce_pairs_kernel(const double* __restrict__ partials,
@@ -541,7 +585,18 @@ __global__ __launch_bounds__(kBlock) void reduce_pairs_kernel(const double* __re
     a += partials[2 * i];
     b += partials[2 * i + 1];
   }
-  block_sum2(a, b, result, 0);
+  block_sum2(a, b, result);
+}
+
+// One block per scan: items[blockIdx.x].out[0..1] = sums of its partial pairs.
+__global__ __launch_bounds__(kBlock) void reduce_batch_kernel(const ReduceItem* __restrict__ items) {
+  const ReduceItem it = items[blockIdx.x];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < it.n; i += kBlock) {
+    a += it.partials[2 * i];
+    b += it.partials[2 * i + 1];
+  }
+  block_sum2(a, b, it.out);
 }
 
 int grid_1d(size_t n) {
@@ -564,22 +619,38 @@ int fp_partial_blocks(const Plan& P, int vc) {
 }
 
 template <typename T>
+FpArgs<T> fp_args(const Plan& P, const T* vol, T* out, int vb, int vs, const float* b,
+                  const float* invRow, double* partials) {
+  return FpArgs<T>{vol,  out,     P.views, P.bnd, P.cols, P.Bz,   P.h2, P.invdB, P.maskf,
+                   P.mext, P.d, vb,    vs,    b,      invRow, partials};
+}
+
+template <typename T>
+BpArgs<T> bp_args(const Plan& P, const T* prj, float* vol, int vb, int vs, int vc,
+                  const float* invCol, uint16_t* shadow) {
+  return BpArgs<T>{prj, vol, P.views, P.src64, P.mask, P.d, P.bc, P.bc64, vb, vs, vc,
+                   invCol, shadow};
+}
+
+int fp_blocks_per_view(const Plan& P, bool colw) {
+  return colw ? (P.d.DNU + kWaves - 1) / kWaves : (P.d.DNU * P.d.DNV + kBlock - 1) / kBlock;
+}
+
+template <typename T>
 int launch_fp(const Plan& P, const T* vol, T* out, int vb, int vs, int vc, hipStream_t st,
               const float* b, const float* invRow, double* partials) {
   if (vc <= 0) return CTR_OK;
+  // with DNV == 64 both mappings have DNU * 64 / 256 blocks per view
   dim3 grid((P.d.DNU * P.d.DNV + kBlock - 1) / kBlock, vc);
   const bool colw = P.d.DNV == 64;
-#define CTR_FP(F, C, B, IR, PA)                                                                 \
-  hipLaunchKernelGGL((fp_kernel<F, C, T>), grid, dim3(kBlock), 0, st, vol, out, P.views, P.bnd, \
-                     P.cols, P.Bz, P.h2, P.invdB, P.maskf, P.mext, P.d, vb, vs, B, IR, PA)
+  const FpArgs<T> A = fp_args<T>(P, vol, out, vb, vs, b, invRow, partials);
   if (b) {
-    if (colw) CTR_FP(true, true, b, invRow, partials);
-    else CTR_FP(true, false, b, invRow, partials);
+    if (colw) hipLaunchKernelGGL((fp_kernel<true, true, T>), grid, dim3(kBlock), 0, st, A);
+    else hipLaunchKernelGGL((fp_kernel<true, false, T>), grid, dim3(kBlock), 0, st, A);
   } else {
-    if (colw) CTR_FP(false, true, nullptr, nullptr, nullptr);
-    else CTR_FP(false, false, nullptr, nullptr, nullptr);
+    if (colw) hipLaunchKernelGGL((fp_kernel<false, true, T>), grid, dim3(kBlock), 0, st, A);
+    else hipLaunchKernelGGL((fp_kernel<false, false, T>), grid, dim3(kBlock), 0, st, A);
   }
-#undef CTR_FP
   return check_launch("fp_kernel");
 }
 
@@ -614,26 +685,65 @@ int launch_bp(const Plan& P, const T* prj, float* vol, int vb, int vs, int vc, i
       return check_launch("bp_matched_kernel");
     }
   }
-#define CTR_BPREF(UPD, ZGV, IC, SH)                                                              \
-  hipLaunchKernelGGL((bp_ref_kernel<UPD, ZGV, T>), rgrid, dim3(kBlock), 0, st, prj, vol, P.views, \
-                     P.src64, P.mask, P.d, P.bc, P.bc64, vb, vs, vc, IC, SH)
+  const BpArgs<T> A = bp_args<T>(P, prj, vol, vb, vs, vc, accumulate ? invCol : nullptr,
+                                 accumulate ? shadow : nullptr);
+#define CTR_BPREF(UPD, ZGV) \
+  hipLaunchKernelGGL((bp_ref_kernel<UPD, ZGV, T>), rgrid, dim3(kBlock), 0, st, A)
   if (accumulate) {
     switch (zg) {
-      case 1: CTR_BPREF(true, 1, invCol, shadow); break;
-      case 2: CTR_BPREF(true, 2, invCol, shadow); break;
-      case 3: CTR_BPREF(true, 3, invCol, shadow); break;
-      default: CTR_BPREF(true, 4, invCol, shadow); break;
+      case 1: CTR_BPREF(true, 1); break;
+      case 2: CTR_BPREF(true, 2); break;
+      case 3: CTR_BPREF(true, 3); break;
+      default: CTR_BPREF(true, 4); break;
     }
   } else {
     switch (zg) {
-      case 1: CTR_BPREF(false, 1, nullptr, nullptr); break;
-      case 2: CTR_BPREF(false, 2, nullptr, nullptr); break;
-      case 3: CTR_BPREF(false, 3, nullptr, nullptr); break;
-      default: CTR_BPREF(false, 4, nullptr, nullptr); break;
+      case 1: CTR_BPREF(false, 1); break;
+      case 2: CTR_BPREF(false, 2); break;
+      case 3: CTR_BPREF(false, 3); break;
+      default: CTR_BPREF(false, 4); break;
     }
   }
 #undef CTR_BPREF
   return check_launch("bp_ref_kernel");
+}
+
+template <typename T>
+int launch_fp_batch(const FpBatchItem<T>* items, int nscan, int total, bool colw, int grid,
+                    hipStream_t st) {
+  if (total <= 0) return CTR_OK;
+  grid = std::max(1, std::min(grid, total));
+  if (colw)
+    hipLaunchKernelGGL((fp_batch_kernel<true, true, T>), dim3(grid), dim3(kBlock), 0, st, items,
+                       nscan, total);
+  else
+    hipLaunchKernelGGL((fp_batch_kernel<true, false, T>), dim3(grid), dim3(kBlock), 0, st, items,
+                       nscan, total);
+  return check_launch("fp_batch_kernel");
+}
+
+template <typename T>
+int launch_bp_batch(const BpBatchItem<T>* items, int nscan, int total, int zg, int grid,
+                    hipStream_t st) {
+  if (total <= 0) return CTR_OK;
+  grid = std::max(1, std::min(grid, total));
+#define CTR_BPB(ZGV)                                                                       \
+  hipLaunchKernelGGL((bp_batch_kernel<true, ZGV, T>), dim3(grid), dim3(kBlock), 0, st, items, \
+                     nscan, total)
+  switch (zg) {
+    case 1: CTR_BPB(1); break;
+    case 2: CTR_BPB(2); break;
+    case 3: CTR_BPB(3); break;
+    case 4: CTR_BPB(4); break;
+    default: return fail(CTR_ERR_ARG, "bp_batch: z groups must be 1..4");
+  }
+#undef CTR_BPB
+  return check_launch("bp_batch_kernel");
+}
+
+int launch_reduce_batch(const ReduceItem* items, int nscan, hipStream_t st) {
+  hipLaunchKernelGGL(reduce_batch_kernel, dim3(nscan), dim3(kBlock), 0, st, items);
+  return check_launch("reduce_batch_kernel");
 }
 
 template <typename T>
@@ -654,6 +764,20 @@ template int launch_bp<float>(const Plan&, const float*, float*, int, int, int, 
                               const float*, int, uint16_t*);
 template int launch_bp<uint16_t>(const Plan&, const uint16_t*, float*, int, int, int, int,
                                  hipStream_t, const float*, int, uint16_t*);
+template FpArgs<float> fp_args<float>(const Plan&, const float*, float*, int, int, const float*,
+                                      const float*, double*);
+template FpArgs<uint16_t> fp_args<uint16_t>(const Plan&, const uint16_t*, uint16_t*, int, int,
+                                            const float*, const float*, double*);
+template BpArgs<float> bp_args<float>(const Plan&, const float*, float*, int, int, int,
+                                      const float*, uint16_t*);
+template BpArgs<uint16_t> bp_args<uint16_t>(const Plan&, const uint16_t*, float*, int, int, int,
+                                            const float*, uint16_t*);
+template int launch_fp_batch<float>(const FpBatchItem<float>*, int, int, bool, int, hipStream_t);
+template int launch_fp_batch<uint16_t>(const FpBatchItem<uint16_t>*, int, int, bool, int,
+                                       hipStream_t);
+template int launch_bp_batch<float>(const BpBatchItem<float>*, int, int, int, int, hipStream_t);
+template int launch_bp_batch<uint16_t>(const BpBatchItem<uint16_t>*, int, int, int, int,
+                                       hipStream_t);
 template int launch_scale_matched<float>(const Plan&, const float*, float*, int, int, int,
                                          hipStream_t);
 template int launch_scale_matched<uint16_t>(const Plan&, const uint16_t*, float*, int, int, int,

@@ -387,9 +387,6 @@ struct DBuf {
 
 using namespace ctr;
 
-struct ctr_plan {
-  Plan* impl;
-};
 
 // An OS-SART session: the subset row / column sums of one measured sinogram, kept so
 // that sweeps can be run in several calls (os_sart.m:14-18 computes them once).

@@ -196,6 +196,40 @@ int ctr_conjgrad(ctr_plan* plan, float* d_x, const float* d_prj, int iterNum, in
 /* Relative data residual ||A x - b|| / ||b|| over all views (FP64 sums). Synchronous. */
 int ctr_residual(ctr_plan* plan, const float* d_x, const float* d_prj, double* out);
 
+/* ---- batched OS-SART over several scans (build addition, no reference counterpart) ----
+ * nscan scans of any sizes (one plan each, same device and storage, direct method),
+ * their iterates d_x[i] and measured sinograms d_prj[i] (kept by the batch until
+ * destroyed).  ctr_batch_create computes every scan's row / column sums and tunes the
+ * launch shapes on the batch (persistent grid sizes, forward work mapping, z groups per
+ * lane of the back projector, chosen by timing); each subset step then runs as one
+ * forward, one reduction and one back-projection launch over all scans, and a whole
+ * sweep is captured once into a HIP graph and replayed.  Reference back projector
+ * only. */
+typedef struct ctr_batch ctr_batch;
+typedef struct ctr_batch_info {
+  int fp_colw;        /* forward mapping: 1 = one wave per detector column, 0 = per ray */
+  int fp_grid;        /* persistent workgroups of the forward launch */
+  int bp_zg;          /* z groups per lane of the back projector */
+  int bp_grid;        /* persistent workgroups of the back-projection launch */
+  double setup_ms;    /* row / column sums of all scans */
+  double tune_ms;     /* autotuning (host wall time) */
+  double capture_ms;  /* graph capture + instantiation (host wall time; 0 before) */
+  long long fp_items, bp_items;  /* work items per sweep */
+} ctr_batch_info;
+int ctr_batch_create(ctr_plan* const* plans, int nscan, float* const* d_x,
+                     const float* const* d_prj, int osNum, int bp_kind, ctr_batch** out);
+/* Up to maxIter sweeps; stops after the first sweep at which every scan's residual is
+ * <= stop_resid (stop_resid <= 0: never).  resid (host, may be NULL) receives
+ * [sweep][scan] residuals, sweep_ms the per-sweep GPU time, done the sweeps run. */
+int ctr_batch_sweeps(ctr_batch* batch, int maxIter, double stop_resid, double* resid,
+                     double* sweep_ms, int* done);
+/* Graph replay on (default) or plain launches. */
+int ctr_batch_set_graph(ctr_batch* batch, int enable);
+/* Override the tuned launch shapes (tests and experiments). */
+int ctr_batch_set_tuning(ctr_batch* batch, int fp_colw, int fp_grid, int bp_zg, int bp_grid);
+int ctr_batch_get_info(const ctr_batch* batch, ctr_batch_info* info);
+int ctr_batch_destroy(ctr_batch* batch);
+
 /* ---- learned warm start (build addition, no reference counterpart) ----
  * A 3-level 3-D U-Net (16/32/64 channels, 3x3x3 convolutions, ReLU, 2x2x2 average
  * pooling, nearest upsampling, concatenated skips, 1x1x1 head) run with BF16
