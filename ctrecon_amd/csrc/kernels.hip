@@ -21,6 +21,11 @@
 namespace ctr {
 namespace {
 
+// A lane's float as a wave-uniform value (v_readlane_b32 into a scalar register).
+__device__ __forceinline__ float rdl(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
 // Block-wide sum of two doubles into slot[0], slot[1].
 __device__ __forceinline__ void block_sum2(double a, double b, double* slot) {
   __shared__ double red[2][kWaves];
@@ -236,54 +241,81 @@ __device__ __forceinline__ void bp_ref_body(const BpArgs<T>& A, int col, int zbl
   const float DNVh = (float)d.DNV - 0.5f;
   const int DNV = d.DNV;
   const uint32_t frame = (uint32_t)(d.DNU * d.DNV);
-  for (int k = 0; k < A.vc; ++k) {
-    const int p = A.vb + k * A.vs;
-    const ViewP vp = A.views[p];
-    // row shadow, FP32 (DD3_GPU.cu:1446-1450): t = (dZ +- hz) * S / l + cV; first the
-    // whole column's row range (uniform), to skip views it does not meet
-    const float dXf = Pxf - vp.sx, dYf = Pyf - vp.sy;
-    const float l2f = dXf * dXf + dYf * dYf;
-    const float rlS = __builtin_amdgcn_rsqf(l2f) * c.S;
-    const bool rneg = rlS < 0.f;  // rows then run backwards: swap the bounds
-    {
+  // Views are taken 64 at a time: lane l first evaluates everything of view k0 + l that
+  // is uniform over the column (visibility, fan-angle shadow in FP64, column weights,
+  // row-shadow constants); the loop over the visible views then reads them with
+  // readlane into scalar registers.  The arithmetic per view is unchanged.
+  for (int k0 = 0; k0 < A.vc; k0 += 64) {
+    const int kl = k0 + lane;
+    float q_rlS = 0.f, q_cA = 0.f, q_cB = 0.f, q_sz = 0.f, q_l2 = 0.f, q_cws = 0.f;
+    float q_wu0 = 0.f, q_fh = 0.f;
+    int q_u0 = 0, q_nu = 0;
+    bool vis = false;
+    if (kl < A.vc) {
+      const int p = A.vb + kl * A.vs;
+      const ViewP vp = A.views[p];
+      // row shadow, FP32 (DD3_GPU.cu:1446-1450): t = (dZ +- hz) * S / l + cV; first the
+      // whole column's row range, to skip views it does not meet
+      const float dXf = Pxf - vp.sx, dYf = Pyf - vp.sy;
+      const float l2f = dXf * dXf + dYf * dYf;
+      const float rlS = __builtin_amdgcn_rsqf(l2f) * c.S;
+      const bool rneg = rlS < 0.f;  // rows then run backwards: swap the bounds
       const float ta = fmaf(Zlo - vp.sz - c.hz, rlS, c.cV);
       const float tb = fmaf(Zhi - vp.sz + c.hz, rlS, c.cV);
-      if (!(fmaxf(ta, tb) > -0.5f && fminf(ta, tb) < DNVh)) continue;
+      const bool zvis = fmaxf(ta, tb) > -0.5f && fminf(ta, tb) < DNVh;
+      // fan-angle shadow, FP64 (DD3_GPU.cu:1447-1464); sy*dX - sx*dY == sy*Px - sx*Py
+      const double2 s64 = A.src64[p];
+      const double dX = Px - s64.x, dY = Py - s64.y;
+      const double l2 = dX * dX + dY * dY;
+      const double rl = rsqrt(l2);
+      const double alpha = asin_fan((s64.y * Px - s64.x * Py) * c64.inv_sid * rl);
+      const double w = vp.wx ? dX : dY;
+      const double dA = w * (rl * rl) * (c64.dx * 0.5);
+      const double tL = (alpha - dA) * c64.ib + c64.cU;
+      const double tR = (alpha + dA) * c64.ib + c64.cU;
+      const double ulo = fmax(fmin(tL, tR), -0.5);
+      const double uhi = fmin(fmax(tL, tR), DNUh);
+      vis = zvis && ulo < uhi;
+      if (vis) {
+        const int u0 = (int)floor(ulo + 0.5);
+        const int nu = min((int)floor(uhi + 0.5), d.DNU - 1) - u0 + 1;
+        // column weights: first / last partial, interior 1 (fractions taken in FP64)
+        const float fl = (float)(((double)u0 + 0.5) - ulo);
+        q_fh = (float)(uhi - ((double)(u0 + nu - 1) - 0.5));
+        q_wu0 = (nu == 1) ? (float)(uhi - ulo) : fl;
+        // sign of the reference's corner difference, (tR - tL) and (tU - tD), folded
+        // into the cell-width factor (a sign flip is exact)
+        const float sgn = ((tR < tL) != rneg) ? -1.f : 1.f;
+        q_cws = sgn * (c.dx * __builtin_amdgcn_rcpf((float)w));
+        const float cUp = fmaf(c.hz, rlS, c.cV), cDp = fmaf(-c.hz, rlS, c.cV);
+        q_cA = rneg ? cUp : cDp;  // lower row bound
+        q_cB = rneg ? cDp : cUp;  // upper row bound
+        q_rlS = rlS;
+        q_sz = vp.sz;
+        q_l2 = l2f;
+        q_u0 = u0;
+        q_nu = nu;
+      }
     }
-    // fan-angle shadow, FP64 (DD3_GPU.cu:1447-1464); sy*dX - sx*dY == sy*Px - sx*Py
-    const double2 s64 = A.src64[p];
-    const double dX = Px - s64.x, dY = Py - s64.y;
-    const double l2 = dX * dX + dY * dY;
-    const double rl = rsqrt(l2);
-    const double alpha = asin_fan((s64.y * Px - s64.x * Py) * c64.inv_sid * rl);
-    const double w = vp.wx ? dX : dY;
-    const double dA = w * (rl * rl) * (c64.dx * 0.5);
-    const double tL = (alpha - dA) * c64.ib + c64.cU;
-    const double tR = (alpha + dA) * c64.ib + c64.cU;
-    const double ulo = fmax(fmin(tL, tR), -0.5);
-    const double uhi = fmin(fmax(tL, tR), DNUh);
-    if (!(ulo < uhi)) continue;
-    const int u0 = (int)floor(ulo + 0.5);
-    const int nu = min((int)floor(uhi + 0.5), d.DNU - 1) - u0 + 1;
-    // column weights: first / last partial, interior 1 (fractions taken in FP64)
-    const float fl = (float)(((double)u0 + 0.5) - ulo);
-    const float fh = (float)(uhi - ((double)(u0 + nu - 1) - 0.5));
-    const float wu0 = (nu == 1) ? (float)(uhi - ulo) : fl;
-    const float cw = c.dx * __builtin_amdgcn_rcpf((float)w);
-    // sign of the reference's corner difference: (tR - tL) and (tU - tD)
-    const float sgn = ((tR < tL) != rneg) ? -1.f : 1.f;
-    const float cUp = fmaf(c.hz, rlS, c.cV), cDp = fmaf(-c.hz, rlS, c.cV);
-    const Buf<T> pp(A.prj + (size_t)k * frame, frame);
+    uint64_t live = __ballot(vis);
+    while (live) {
+      const int j = __builtin_ctzll(live);
+      live &= live - 1;
+      const int k = k0 + j;
+      const float rlS = rdl(q_rlS, j), cA = rdl(q_cA, j), cB = rdl(q_cB, j);
+      const float vsz = rdl(q_sz, j), l2f = rdl(q_l2, j), cws = rdl(q_cws, j);
+      const float wu0 = rdl(q_wu0, j), fh = rdl(q_fh, j);
+      const int u0 = __builtin_amdgcn_readlane(q_u0, j), nu = __builtin_amdgcn_readlane(q_nu, j);
+      const Buf<T> pp(A.prj + (size_t)k * frame, frame);
 #pragma unroll
-    for (int g = 0; g < ZG; ++g) {
-      const float dZ = fmaf((float)g, dzg, Pz0) - vp.sz;
-      const float tU = fmaf(dZ, rlS, cUp), tD = fmaf(dZ, rlS, cDp);
-      const float vlo = fmaxf(rneg ? tU : tD, -0.5f);
-      const float vhi = fminf(rneg ? tD : tU, DNVh);
+      for (int g = 0; g < ZG; ++g) {
+      const float dZ = fmaf((float)g, dzg, Pz0) - vsz;
+      const float vlo = fmaxf(fmaf(dZ, rlS, cA), -0.5f);
+      const float vhi = fminf(fmaf(dZ, rlS, cB), DNVh);
       const bool vok = zb + 64 * g < d.ZN && vlo < vhi;
       if (!__any(vok)) continue;
       if (!vok) continue;
-      const float cosVal = cw * __builtin_amdgcn_sqrtf(fmaf(dZ, dZ, l2f));
+      const float cosVal = cws * __builtin_amdgcn_sqrtf(fmaf(dZ, dZ, l2f));
       const int v0 = (int)floorf(vlo + 0.5f);
       const int nv = min((int)floorf(vhi + 0.5f), DNV - 1) - v0 + 1;
       float wv0, wvl;
@@ -317,7 +349,8 @@ __device__ __forceinline__ void bp_ref_body(const BpArgs<T>& A, int col, int zbl
           s = fmaf(wu, gsum, s);
         }
       }
-      acc[g] = fmaf(sgn * cosVal, s, acc[g]);
+      acc[g] = fmaf(cosVal, s, acc[g]);
+      }
     }
   }
 #pragma unroll

@@ -86,7 +86,8 @@ def _axpy(d_y, d_x, a):
 
 
 def make_pair(plan: Plan, vol, iters=50, osNum=20, kind="ref"):
-    """(x2, delta, target) float32 host arrays from plain OS-SART on b = A vol."""
+    """(x2, delta, x_K - x2, vol - x2) float32 host arrays from plain OS-SART on
+    b = A vol: the network input and two targets (the K-sweep iterate, the phantom)."""
     g = plan.g
     d_b = DeviceArray(g.prj_shape)
     plan.fp(DeviceArray.from_host(vol), d_b)
@@ -100,7 +101,7 @@ def make_pair(plan: Plan, vol, iters=50, osNum=20, kind="ref"):
     S.sweeps(dx, iters - 2)
     xk = dx.get()
     S.close()
-    return x2, x2 - x1, xk - x2
+    return x2, x2 - x1, xk - x2, np.asarray(vol, np.float32) - x2
 
 
 def _cosine(step, steps, lr):
@@ -233,7 +234,12 @@ def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m ctrecon_amd.learned")
     sub = ap.add_subparsers(dest="cmd", required=True)
     tr = sub.add_parser("train", help="generate pairs with plain OS-SART and train")
-    tr.add_argument("--out", required=True)
+    tr.add_argument("--out", required=True, help="weights file; with several targets, "
+                    "one file per target: <out>_<target>.npz")
+    tr.add_argument("--targets", default="iter", help="comma list of 'iter' (the K-sweep "
+                    "OS-SART iterate) and 'truth' (the phantom)")
+    tr.add_argument("--eval-seeds", type=int, nargs="*", default=[])
+    tr.add_argument("--target", type=float, default=1e-3)
     tr.add_argument("--phantoms", type=int, default=64)
     tr.add_argument("--val", type=int, default=4)
     tr.add_argument("--seed0", type=int, default=1000)
@@ -274,9 +280,25 @@ def main(argv=None):
         pairs.append(make_pair(plan, vol, a.iters, a.subsets))
         _log(f"pair {i + 1}/{len(vols)} ({time.time() - t0:.0f} s)")
     del vols
-    net = train(pairs[:a.phantoms], a.steps, lr=a.lr, val=pairs[a.phantoms:])
-    net.save(a.out)
-    _log(f"saved {a.out} ({time.time() - t0:.0f} s)")
+    targets = [t.strip() for t in a.targets.split(",") if t.strip()]
+    nets = {}
+    for tname in targets:
+        ti = {"iter": 2, "truth": 3}[tname]
+        sel = [(p[0], p[1], p[ti]) for p in pairs]
+        _log(f"training on target '{tname}'")
+        net = train(sel[:a.phantoms], a.steps, lr=a.lr, val=sel[a.phantoms:])
+        out = a.out if len(targets) == 1 else a.out.replace(".npz", f"_{tname}.npz")
+        net.save(out)
+        nets[tname] = net
+        _log(f"saved {out} ({time.time() - t0:.0f} s)")
+    del pairs
+    for seed in a.eval_seeds:
+        vol = training_phantom(g, seed)
+        for tname, net in nets.items():
+            r = compare(plan, net, vol, a.target, a.subsets)
+            print(f"eval seed {seed} target {tname}: " + ", ".join(
+                f"{k} {v:.4g}" if isinstance(v, float) else f"{k} {v}" for k, v in r.items()),
+                flush=True)
 
 
 if __name__ == "__main__":
