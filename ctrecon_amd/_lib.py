@@ -31,7 +31,7 @@ _NAMES = {0: "CTR_OK", 1: "CTR_ERR_ARG", 2: "CTR_ERR_HIP", 3: "CTR_ERR_UNSUPPORT
 # Every symbol include/ctrecon.h declares.
 EXPORTS = [
     "ctr_dd3_proj", "ctr_dd3_back", "ctr_plan_create", "ctr_plan_destroy", "ctr_plan_set_method",
-    "ctr_plan_set_storage", "ctr_plan_fp", "ctr_plan_fp_bf16", "ctr_plan_bp_bf16",
+    "ctr_plan_set_storage", "ctr_plan_mask_volume", "ctr_plan_fp", "ctr_plan_fp_bf16", "ctr_plan_bp_bf16",
     "ctr_f32_to_bf16", "ctr_bf16_to_f32",
     "ctr_plan_bp", "ctr_os_sart", "ctr_os_sart_run", "ctr_cone_voxels", "ctr_conjgrad",
     "ctr_residual", "ctr_device_count",
@@ -100,6 +100,7 @@ def lib():
         L.ctr_plan_destroy.argtypes = [p]
         L.ctr_plan_set_method.argtypes = [p, i]
         L.ctr_plan_set_storage.argtypes = [p, i]
+        L.ctr_plan_mask_volume.argtypes = [p, p, p]
         L.ctr_plan_fp.argtypes = [p, p, p, i, i, i, p]
         L.ctr_plan_fp_bf16.argtypes = [p, p, p, i, i, i, p]
         L.ctr_plan_bp_bf16.argtypes = [p, p, p, i, i, i, i, p]

@@ -332,7 +332,7 @@ int ctr_batch_create(ctr_plan* const* plans, int nscan, float* const* d_x,
   CTR_HIP(hipMemsetAsync(B->zeros.p, 0, maxNV * sizeof(float), B->st));
   if (B->bf)
     for (auto& c : B->scans)
-      if ((rc = launch_to_bf16(c->xs.h(), c->x, c->NV, B->st))) return rc;
+      if ((rc = launch_to_bf16_shadow(c->xs.h(), c->x, c->NV, B->st))) return rc;
   const auto t0 = std::chrono::steady_clock::now();
   if ((rc = autotune(*B))) return rc;
   CTR_HIP(hipStreamSynchronize(B->st));
@@ -391,7 +391,7 @@ int ctr_batch_sweeps(ctr_batch* B, int maxIter, double stop_resid, double* resid
   // the caller may have changed the iterates: refresh the BF16 shadows
   if (B->bf)
     for (auto& c : B->scans)
-      if ((rc = launch_to_bf16(c->xs.h(), c->x, c->NV, B->st))) return rc;
+      if ((rc = launch_to_bf16_shadow(c->xs.h(), c->x, c->NV, B->st))) return rc;
   if (B->useGraph && !B->exec) {
     const auto t0 = std::chrono::steady_clock::now();
     CTR_HIP(hipStreamBeginCapture(B->st, hipStreamCaptureModeThreadLocal));

@@ -191,7 +191,10 @@ int launch_bp_batch(const BpBatchItem<T>* items, int nscan, int total, int zg, i
                     hipStream_t st);
 int launch_reduce_batch(const ReduceItem* items, int nscan, hipStream_t st);
 int launch_to_bf16(uint16_t* dst, const float* src, size_t n, hipStream_t st);
+// the solvers' dithered BF16 shadow of an iterate (to_bf16_dither)
+int launch_to_bf16_shadow(uint16_t* dst, const float* src, size_t n, hipStream_t st);
 int launch_from_bf16(float* dst, const uint16_t* src, size_t n, hipStream_t st);
+int launch_mask_volume(const Plan& P, float* vol, hipStream_t st);
 int launch_fill(float* p, size_t n, float v, hipStream_t st);
 int launch_copy(float* dst, const float* src, size_t n, hipStream_t st);
 int launch_inv_guard(const float* in, float* out, size_t n, hipStream_t st);

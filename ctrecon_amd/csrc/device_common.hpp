@@ -28,6 +28,17 @@ __device__ __forceinline__ uint16_t to_bf16(float v) {
 }
 __device__ __forceinline__ void st(uint16_t* p, size_t i, float v) { p[i] = to_bf16(v); }
 
+// BF16 with a fixed per-element dither, for the solvers' shadow of the iterate: the 16
+// dropped bits are raised by r(i) = frac(i * golden ratio) * 2^16 before truncation, so
+// element i rounds up with probability (dropped bits) / 2^16 over a region.  Round to
+// nearest gives every voxel of a flat region the same error, which a projection sums
+// (a data-residual floor near the BF16 precision); dithered errors average out along a
+// ray.  |error| < 1 BF16 ulp.
+__device__ __forceinline__ uint16_t to_bf16_dither(float v, size_t i) {
+  const uint32_t r = ((uint32_t)i * 2654435769u) >> 16;
+  return (uint16_t)((__float_as_uint(v) + r) >> 16);
+}
+
 // Raw buffer view of a float or BF16 array: 32-bit byte offsets, and reads past
 // num_records return 0 instead of touching memory outside the array.
 template <typename T>

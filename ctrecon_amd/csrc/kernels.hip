@@ -93,23 +93,13 @@ __device__ __forceinline__ void fp_body(const FpArgs<T>& A, int k, int ublk, dou
     const Buf<T> V(A.vol, (uint32_t)(d.XN * d.YN * d.ZN));
     const float* maskf = A.maskf;
     float acc = 0.f;
-    for (int ii = 0; ii < NS; ++ii) {
-      const float fi = (float)ii;
-      const float tl = fmaf(fi, bl.y, bl.x);
-      const float tr = fmaf(fi, br.y, br.x);
-      float ylo = fminf(tl, tr), yhi = fmaxf(tl, tr);
-      // clip to the mask's extent on this slice (inside [-0.5, NI - 0.5]); cells
-      // outside it are zero
-      const float2 e = me[ii];
-      const float clo = fmaxf(ylo, e.x), chi = fminf(yhi, e.y);
-      if (!(clo < chi)) continue;
-      const float r = fmaf(fi, ct.y, ct.x);
-      const float inv = __builtin_amdgcn_rcpf((yhi - ylo) * r);
+    // one slice: footprint [clo, chi] x [zl, zh] over the cells j0 .. j0 + nj - 1
+    auto slice = [&](int ii, float r, float inv, int j0, int nj, float wj0, float wjl) {
       float zl = fmaf(r, Bv, vp.az), zh = fmaf(r, Bv1, vp.az);
       zl = fmaxf(zl, -0.5f);
       zh = fminf(zh, ZNh);
       const bool zok = zl < zh;
-      if (!__any(zok)) continue;
+      if (!__any(zok)) return;
       const int k0 = min((int)floorf(zl + 0.5f), ZN1);
       const int nk = zok ? min((int)floorf(zh + 0.5f), ZN1) - k0 + 1 : 0;
       float wz0, wzl;
@@ -117,10 +107,6 @@ __device__ __forceinline__ void fp_body(const FpArgs<T>& A, int k, int ublk, dou
       const float wz1 = (nk == 2) ? wzl : (nk >= 3 ? 1.f : 0.f);
       const float wz2 = (nk == 3) ? wzl : 0.f;
       if (nk == 0) wz0 = 0.f;
-      const int j0 = (int)floorf(clo + 0.5f);
-      const int nj = min((int)floorf(chi + 0.5f), NI - 1) - j0 + 1;
-      float wj0, wjl;
-      edge_weights(clo, chi, j0, nj, wj0, wjl);
       int o = ii * sS + j0 * cS + k0;
       int mo = ii * mS + j0 * mC;
       // unused taps re-read the first cell (weight 0), never a neighbouring column
@@ -148,6 +134,60 @@ __device__ __forceinline__ void fp_body(const FpArgs<T>& A, int k, int ublk, dou
         }
       }
       acc = fmaf(s, inv, acc);
+    };
+    if constexpr (COLW) {
+      // the in-plane part of a slice is uniform over the wave: lane l evaluates it for
+      // slice ii0 + l, and the loop over the slices the footprint meets reads it back
+      // with readlane into scalar registers (same arithmetic per slice)
+      const int lane = threadIdx.x & 63;
+      for (int ii0 = 0; ii0 < NS; ii0 += 64) {
+        const int il = ii0 + lane;
+        float q_r = 0.f, q_inv = 0.f, q_wj0 = 0.f, q_wjl = 0.f;
+        int q_j0 = 0, q_nj = 0;
+        bool ok = false;
+        if (il < NS) {
+          const float fi = (float)il;
+          const float tl = fmaf(fi, bl.y, bl.x);
+          const float tr = fmaf(fi, br.y, br.x);
+          const float ylo = fminf(tl, tr), yhi = fmaxf(tl, tr);
+          // clip to the mask's extent on this slice (inside [-0.5, NI - 0.5]); cells
+          // outside it are zero
+          const float2 e = me[il];
+          const float clo = fmaxf(ylo, e.x), chi = fminf(yhi, e.y);
+          ok = clo < chi;
+          if (ok) {
+            q_r = fmaf(fi, ct.y, ct.x);
+            q_inv = __builtin_amdgcn_rcpf((yhi - ylo) * q_r);
+            q_j0 = (int)floorf(clo + 0.5f);
+            q_nj = min((int)floorf(chi + 0.5f), NI - 1) - q_j0 + 1;
+            edge_weights(clo, chi, q_j0, q_nj, q_wj0, q_wjl);
+          }
+        }
+        uint64_t live = __ballot(ok);
+        while (live) {
+          const int j = __builtin_ctzll(live);
+          live &= live - 1;
+          slice(ii0 + j, rdl(q_r, j), rdl(q_inv, j), __builtin_amdgcn_readlane(q_j0, j),
+                __builtin_amdgcn_readlane(q_nj, j), rdl(q_wj0, j), rdl(q_wjl, j));
+        }
+      }
+    } else {
+      for (int ii = 0; ii < NS; ++ii) {
+        const float fi = (float)ii;
+        const float tl = fmaf(fi, bl.y, bl.x);
+        const float tr = fmaf(fi, br.y, br.x);
+        const float ylo = fminf(tl, tr), yhi = fmaxf(tl, tr);
+        const float2 e = me[ii];
+        const float clo = fmaxf(ylo, e.x), chi = fminf(yhi, e.y);
+        if (!(clo < chi)) continue;
+        const float r = fmaf(fi, ct.y, ct.x);
+        const float inv = __builtin_amdgcn_rcpf((yhi - ylo) * r);
+        const int j0 = (int)floorf(clo + 0.5f);
+        const int nj = min((int)floorf(chi + 0.5f), NI - 1) - j0 + 1;
+        float wj0, wjl;
+        edge_weights(clo, chi, j0, nj, wj0, wjl);
+        slice(ii, r, inv, j0, nj, wj0, wjl);
+      }
     }
     const float K = ct.w * __builtin_amdgcn_sqrtf(ct.z + A.h2[v]) * A.invdB[v];
     const float val = acc * K;
@@ -360,7 +400,7 @@ __device__ __forceinline__ void bp_ref_body(const BpArgs<T>& A, int col, int zbl
     if constexpr (UPDATE) {
       const float nx = fmaf(acc[g], A.invCol[(size_t)col * d.ZN + z], vcol[z]);
       vcol[z] = nx;
-      if (A.shadow) A.shadow[(size_t)col * d.ZN + z] = to_bf16(nx);
+      if (A.shadow) A.shadow[(size_t)col * d.ZN + z] = to_bf16_dither(nx, (size_t)col * d.ZN + z);
     } else {
       vcol[z] = acc[g];
     }
@@ -518,7 +558,7 @@ __global__ __launch_bounds__(kBlock) void bp_matched_kernel(
     if constexpr (UPDATE) {
       const float nx = fmaf(a, invCol[(size_t)col * d.ZN + z], vcol[z]);
       vcol[z] = nx;
-      if (shadow) shadow[(size_t)col * d.ZN + z] = to_bf16(nx);
+      if (shadow) shadow[(size_t)col * d.ZN + z] = to_bf16_dither(nx, (size_t)col * d.ZN + z);
     } else {
       vcol[z] = a;
     }
@@ -559,6 +599,21 @@ __global__ void to_bf16_kernel(uint16_t* __restrict__ dst, const float* __restri
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
     dst[i] = to_bf16(src[i]);
+}
+
+__global__ void to_bf16_dither_kernel(uint16_t* __restrict__ dst, const float* __restrict__ src,
+                                      size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = to_bf16_dither(src[i], i);
+}
+
+// vol[col][z] *= mask[col] (the in-place masking of DD3_GPU.cu:1004-1011, on the device).
+__global__ void mask_volume_kernel(float* __restrict__ vol, const float* __restrict__ maskf,
+                                   int ZN, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    vol[i] *= maskf[i / ZN];
 }
 
 __global__ void from_bf16_kernel(float* __restrict__ dst, const uint16_t* __restrict__ src,
@@ -819,6 +874,18 @@ template int launch_scale_matched<uint16_t>(const Plan&, const uint16_t*, float*
 int launch_to_bf16(uint16_t* dst, const float* src, size_t n, hipStream_t st) {
   hipLaunchKernelGGL(to_bf16_kernel, dim3(grid_1d(n)), dim3(256), 0, st, dst, src, n);
   return check_launch("to_bf16_kernel");
+}
+
+int launch_to_bf16_shadow(uint16_t* dst, const float* src, size_t n, hipStream_t st) {
+  hipLaunchKernelGGL(to_bf16_dither_kernel, dim3(grid_1d(n)), dim3(256), 0, st, dst, src, n);
+  return check_launch("to_bf16_dither_kernel");
+}
+
+int launch_mask_volume(const Plan& P, float* vol, hipStream_t st) {
+  const size_t n = (size_t)P.d.XN * P.d.YN * P.d.ZN;
+  hipLaunchKernelGGL(mask_volume_kernel, dim3(grid_1d(n)), dim3(256), 0, st, vol, P.maskf,
+                     P.d.ZN, n);
+  return check_launch("mask_volume_kernel");
 }
 
 int launch_from_bf16(float* dst, const uint16_t* src, size_t n, hipStream_t st) {

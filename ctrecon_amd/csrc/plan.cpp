@@ -520,6 +520,13 @@ int ctr_plan_set_storage(ctr_plan* plan, int storage) {
   return CTR_OK;
 }
 
+int ctr_plan_mask_volume(ctr_plan* plan, float* d_vol, void* stream) {
+  Plan* P = get(plan);
+  if (!P || !d_vol) return fail(CTR_ERR_ARG, "NULL plan or buffer");
+  CTR_HIP(hipSetDevice(P->device));
+  return launch_mask_volume(*P, d_vol, (hipStream_t)stream);
+}
+
 int ctr_plan_fp_bf16(ctr_plan* plan, const unsigned short* d_vol, unsigned short* d_prj,
                      int view_begin, int view_stride, int view_count, void* stream) {
   Plan* P = get(plan);
@@ -689,7 +696,7 @@ int ctr_sart_sweeps(ctr_sart* S, float* d_x, int maxIter, double stop_resid, dou
   }
   if (timing) CTR_HIP(hipEventRecord(ev.e[0], st));
   // the caller may have changed x since the last call: refresh its BF16 shadow
-  if (S->bf && (rc = launch_to_bf16(S->xs.h(), d_x, NV, st))) return rc;
+  if (S->bf && (rc = launch_to_bf16_shadow(S->xs.h(), d_x, NV, st))) return rc;
   hipEvent_t* kev = timing ? ev.e.data() + 1 + maxIter : nullptr;
   int done = 0;
   for (int it = 0; it < maxIter; ++it) {
@@ -804,7 +811,7 @@ int ctr_conjgrad(ctr_plan* plan, float* d_x, const float* d_prj, int iterNum, in
   hipStream_t st = nullptr;
   auto A = [&](const float* in, float* out) -> int {  // Back(Proj(.)) (conjgrad.m:4-6)
     if (bf) {
-      int e = launch_to_bf16(in16.h(), in, NV, st);
+      int e = launch_to_bf16_shadow(in16.h(), in, NV, st);
       if (e) return e;
       if ((e = fp_any<uint16_t>(*P, in16.h(), sino.h(), 0, 1, d.PN, st, nullptr, nullptr,
                                 nullptr)))

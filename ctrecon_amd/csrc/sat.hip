@@ -249,7 +249,7 @@ __global__ __launch_bounds__(kBlock) void bp_sat_kernel(
   if constexpr (UPDATE) {
     const float nx = fmaf((float)acc, invCol[o], vol[o]);
     vol[o] = nx;
-    if (shadow) shadow[o] = to_bf16(nx);
+    if (shadow) shadow[o] = to_bf16_dither(nx, o);
   } else {
     vol[o] = (float)acc;
   }

@@ -35,10 +35,15 @@ def _log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def perlin3(shape, period, rng) -> np.ndarray:
+def perlin3(shape, period, rng, step=1) -> np.ndarray:
     """Gradient (Perlin) noise on a grid: random unit gradients on a lattice of spacing
     ``period`` voxels, quintic fade, trilinear blend of the corner dot products.
-    Values lie in about [-1, 1]."""
+    Values lie in about [-1, 1].  step > 1 evaluates it every ``step`` voxels and
+    repeats the values (large grids)."""
+    if step > 1:
+        coarse = perlin3(tuple(-(-n // step) for n in shape), period / step, rng)
+        full = coarse.repeat(step, 0).repeat(step, 1).repeat(step, 2)
+        return np.ascontiguousarray(full[:shape[0], :shape[1], :shape[2]])
     Y, X, Z = shape
     n = [int(math.ceil(s / period)) + 2 for s in shape]
     gr = rng.normal(size=(n[0], n[1], n[2], 3)).astype(np.float32)
@@ -67,12 +72,18 @@ def perlin3(shape, period, rng) -> np.ndarray:
 
 
 def training_phantom(g, seed: int) -> np.ndarray:
-    """Ellipsoid mixture (densities 0-2) plus a two-octave Perlin texture inside the
-    support, clipped to [0, 2]; float32 vol[y, x, z]."""
-    vol = PH.ellipsoid_mixture(g, seed=seed, n=14, axes_xy=(4.0, 45.0), axes_z=(4.0, 30.0),
-                               texture_sigma=0.01)
+    """Ellipsoid mixture (densities 0-2; 40 ellipsoids of 5-60 mm in-plane / 5-40 mm in z
+    on clinical-size grids, 14 of 4-45 / 4-30 mm on smaller ones) plus a two-octave
+    Perlin texture inside the support, clipped to [0, 2]; float32 vol[y, x, z]."""
+    big = g.XN * g.YN * g.ZN > (1 << 25)
+    if big:
+        vol = PH.ellipsoid_mixture(g, seed=seed, n=40, texture_sigma=0.01)
+    else:
+        vol = PH.ellipsoid_mixture(g, seed=seed, n=14, axes_xy=(4.0, 45.0),
+                                   axes_z=(4.0, 30.0), texture_sigma=0.01)
     rng = np.random.default_rng(seed + 7919)
-    tex = 0.08 * perlin3(g.vol_shape, 24, rng) + 0.04 * perlin3(g.vol_shape, 8, rng)
+    st = 2 if big else 1
+    tex = 0.08 * perlin3(g.vol_shape, 24, rng, st) + 0.04 * perlin3(g.vol_shape, 8, rng, st)
     vol = np.where(vol > 0, vol + tex * np.maximum(vol, 0.25), 0.0)
     return np.clip(vol, 0.0, 2.0).astype(np.float32)
 
@@ -98,10 +109,12 @@ def make_pair(plan: Plan, vol, iters=50, osNum=20, kind="ref"):
     x1 = dx.get()
     S.sweeps(dx, 1)
     x2 = dx.get()
-    S.sweeps(dx, iters - 2)
+    if iters > 2:  # the K-sweep target (not needed when training on the phantom only)
+        S.sweeps(dx, iters - 2)
     xk = dx.get()
     S.close()
-    return x2, x2 - x1, xk - x2, np.asarray(vol, np.float32) - x2
+    return (x2, x2 - x1, xk - x2 if iters > 2 else None,
+            np.asarray(vol, np.float32) - x2)
 
 
 def _cosine(step, steps, lr):
@@ -166,6 +179,7 @@ def warm_start_os_sart(plan: Plan, net: UNet, d_b: DeviceArray, d_x: DeviceArray
     delta = DeviceArray(shape)
     _sub(delta, d_x, x1)
     net.forward(d_x, delta, x1)  # x1 now holds the predicted update
+    plan.mask_volume(x1)  # no update outside the FOV mask (those voxels stay 0)
     delta.copy_from(d_x)  # delta now holds x2 (kept for the fallback)
     _axpy(d_x, x1, 1.0)
     plan.synchronize()
@@ -238,7 +252,10 @@ def main(argv=None):
                     "one file per target: <out>_<target>.npz")
     tr.add_argument("--targets", default="iter", help="comma list of 'iter' (the K-sweep "
                     "OS-SART iterate) and 'truth' (the phantom)")
-    tr.add_argument("--eval-seeds", type=int, nargs="*", default=[])
+    tr.add_argument("--eval-seeds", type=int, nargs="*", default=[],
+                    help="held-out training-recipe phantoms to evaluate on")
+    tr.add_argument("--eval-mixture-seeds", type=int, nargs="*", default=[],
+                    help="held-out plain ellipsoid mixtures (the benchmark's recipe)")
     tr.add_argument("--target", type=float, default=1e-3)
     tr.add_argument("--phantoms", type=int, default=64)
     tr.add_argument("--val", type=int, default=4)
@@ -292,8 +309,10 @@ def main(argv=None):
         nets[tname] = net
         _log(f"saved {out} ({time.time() - t0:.0f} s)")
     del pairs
-    for seed in a.eval_seeds:
-        vol = training_phantom(g, seed)
+    evals = [(s_, lambda s_=s_: training_phantom(g, s_)) for s_ in a.eval_seeds]
+    evals += [(s_, lambda s_=s_: PH.ellipsoid_mixture(g, seed=s_)) for s_ in a.eval_mixture_seeds]
+    for seed, make in evals:
+        vol = make()
         for tname, net in nets.items():
             r = compare(plan, net, vol, a.target, a.subsets)
             print(f"eval seed {seed} target {tname}: " + ", ".join(

@@ -168,6 +168,10 @@ class Plan:
         check(lib().ctr_plan_set_storage(self.handle, code))
         self.storage = storage
 
+    def mask_volume(self, d_vol: DeviceArray, stream=None):
+        """d_vol *= mask (per voxel column), on the device."""
+        check(lib().ctr_plan_mask_volume(self.handle, d_vol.ptr, stream))
+
     def close(self):
         if getattr(self, "handle", None):
             lib().ctr_plan_destroy(self.handle)

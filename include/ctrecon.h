@@ -130,6 +130,10 @@ int ctr_plan_fp(ctr_plan* plan, const float* d_vol, float* d_prj, int view_begin
 int ctr_plan_bp(ctr_plan* plan, const float* d_prj, float* d_vol, int view_begin,
                 int view_stride, int view_count, int bp_kind, void* stream);
 
+/* d_vol[y][x][z] *= mask[y*XN + x]: the reference's in-place masking of the volume
+ * (iterativeRecon/DD3_GPU.cu:1004-1011) on a device volume.  Asynchronous. */
+int ctr_plan_mask_volume(ctr_plan* plan, float* d_vol, void* stream);
+
 /* ctr_plan_fp / ctr_plan_bp with BF16 storage (build addition): d_vol and d_prj of
  * the forward projector, and d_prj of the back projector, are BF16 arrays; the back
  * projector's output d_vol is FP32.  Asynchronous. */

@@ -1,0 +1,53 @@
+"""What the trained warm-start network gains over plain OS-SART.
+
+A held-out phantom (seed outside the training seeds) on the training geometry (clinical
+detector and trajectory, 256x256x128 at 0.7 mm), BF16 storage: plain OS-SART and the
+warm-started solve (two sweeps, the network's update, plain sweeps, fallback) run to the
+same data-residual target; the warm start must need fewer sweeps and must not take the
+fallback, and its result must stay as close to the phantom as the plain one."""
+
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "ctrecon_amd", "weights", "warm_start.npz")
+
+
+def test_warm_start_saves_sweeps(gpu_plan_factory):
+    from ctrecon_amd import geometry as G
+    from ctrecon_amd import learned as L
+    from ctrecon_amd.plan import DeviceArray, Plan, Sart
+    from ctrecon_amd.unet import UNet
+
+    print("[learned warm start] seed 30000, 256x256x128, target 2e-3", flush=True)
+    assert os.path.exists(WEIGHTS), "trained weights are part of the product"
+    g = G.dd3_geometry(G.config_training(), mask=G.fov_mask(256))
+    vol = L.training_phantom(g, 30000)
+    plan = Plan(g, 0)
+    plan.set_storage("bf16")
+    net = UNet.from_file(WEIGHTS)
+    d_b = DeviceArray(g.prj_shape)
+    plan.fp(DeviceArray.from_host(vol), d_b)
+    target = 2e-3
+    dx = DeviceArray(g.vol_shape)
+    dx.zero()
+    S = Sart(plan, d_b, 20)
+    r_plain = S.sweeps(dx, 300, target)[0]
+    x_plain = dx.get()
+    info = L.warm_start_os_sart(plan, net, d_b, dx, 20, target, 300, sart=S)
+    x_warm = dx.get()
+    m = g.mask.reshape(g.YN, g.XN).astype(bool)
+    rmse_p = float(np.sqrt(np.mean((x_plain - vol)[m] ** 2)))
+    rmse_w = float(np.sqrt(np.mean((x_warm - vol)[m] ** 2)))
+    print(f"  plain: {len(r_plain)} sweeps (final {r_plain[-1]:.3e}, RMSE {rmse_p:.4f}); "
+          f"warm: {info['sweeps']} sweeps (final {info['residuals'][-1]:.3e}, RMSE {rmse_w:.4f}, "
+          f"fallback {info['fallback']}); residual after the warm start "
+          f"{info['residuals'][2]:.3e} vs plain sweep 3 {r_plain[2]:.3e}", flush=True)
+    assert r_plain[-1] <= target and info["residuals"][-1] <= target
+    assert not info["fallback"]
+    assert info["sweeps"] < len(r_plain)
+    assert rmse_w <= 1.01 * rmse_p
