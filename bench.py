@@ -172,42 +172,51 @@ def main():
         log(f"plain FP32: {len(res)} sweeps, wall {wall:.2f} s, residual {res[-1]:.3e}, "
             f"time to target {plain['time_to_target_s']}")
 
-    learned = None
+    learned = learned_sino = None
     if run_learned:
         from ctrecon_amd.learned import warm_start_os_sart
         from ctrecon_amd.unet import UNet
 
         net = UNet.from_file(args.weights)
-        plan.set_storage("bf16")
-        # warm-up: one BF16 sweep and one network pass (code loading, network buffers)
-        d_x.zero()
-        plan.os_sart_run(d_x, d_b, args.subsets, 1, args.bp)
-        tmp = DeviceArray(g.vol_shape)
-        net.forward(d_x, d_x, tmp)
-        plan.synchronize()
-        del tmp
-        d_x.zero()
-        plan.synchronize()
-        t0 = time.perf_counter()
-        info = warm_start_os_sart(plan, net, d_b, d_x, args.subsets, args.target, max_iter,
-                                  args.bp)
-        wall = time.perf_counter() - t0
-        plan.set_storage("f32")
-        r = np.array(info["residuals"])
-        learned = {
-            "time_to_target_s": wall if r[-1] <= args.target else None,
-            "wall_s": wall,
-            "sweeps": int(info["sweeps"]),
-            "fallback_taken": bool(info["fallback"]),
-            "phase_s": {k: info[k] for k in ("setup_s", "pre_sweeps_s", "network_s",
-                                             "post_sweeps_s")},
-            "residual_sweep_final": float(r[-1]),
-            "residual_true_final": plan.residual(d_x, d_b),
-            "rmse_vs_truth_in_mask": rmse_of(d_x),
-            "weights": os.path.relpath(args.weights, ROOT),
-        }
-        log(f"learned + BF16: {info['sweeps']} sweeps, wall {wall:.2f} s, residual {r[-1]:.3e}, "
-            f"fallback {info['fallback']}")
+
+        def learned_leg(storage):
+            plan.set_storage(storage)
+            # warm-up: one sweep and one network pass (code loading, network buffers)
+            d_x.zero()
+            plan.os_sart_run(d_x, d_b, args.subsets, 1, args.bp)
+            tmp = DeviceArray(g.vol_shape)
+            net.forward(d_x, d_x, tmp)
+            plan.synchronize()
+            del tmp
+            d_x.zero()
+            plan.synchronize()
+            t0 = time.perf_counter()
+            info = warm_start_os_sart(plan, net, d_b, d_x, args.subsets, args.target, max_iter,
+                                      args.bp)
+            wall = time.perf_counter() - t0
+            plan.set_storage("f32")
+            r = np.array(info["residuals"])
+            out = {
+                "storage": storage,
+                "time_to_target_s": wall if r[-1] <= args.target else None,
+                "wall_s": wall,
+                "sweeps": int(info["sweeps"]),
+                "fallback_taken": bool(info["fallback"]),
+                "phase_s": {k: info[k] for k in ("setup_s", "pre_sweeps_s", "network_s",
+                                                 "post_sweeps_s")},
+                "residual_sweep_final": float(r[-1]),
+                "residual_true_final": plan.residual(d_x, d_b),
+                "rmse_vs_truth_in_mask": rmse_of(d_x),
+                "weights": os.path.relpath(args.weights, ROOT),
+            }
+            log(f"learned + {storage}: {info['sweeps']} sweeps, wall {wall:.2f} s, "
+                f"residual {r[-1]:.3e}, fallback {info['fallback']}")
+            return out
+
+        # BF16 volume and sinogram operands (the reported form), then BF16 sinogram
+        # operands with the FP32 iterate
+        learned = learned_leg("bf16")
+        learned_sino = learned_leg("bf16_sino")
 
     main_form = learned if learned is not None else plain
     t_target = main_form["time_to_target_s"]
@@ -278,6 +287,7 @@ def main():
         "sweeps_to_target": sweeps_target,
         "plain_fp32": plain,
         "learned_bf16": learned,
+        "learned_bf16_sinogram_only": learned_sino,
         "speedup_vs_plain": (plain["wall_s"] / learned["wall_s"]
                              if plain is not None and learned is not None else None),
         "rmse_ratio_vs_plain": (learned["rmse_vs_truth_in_mask"] / plain["rmse_vs_truth_in_mask"]

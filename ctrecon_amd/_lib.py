@@ -24,6 +24,7 @@ CTR_METHOD_DIRECT = 0
 CTR_METHOD_SAT = 1
 CTR_STORAGE_F32 = 0
 CTR_STORAGE_BF16 = 1
+CTR_STORAGE_BF16_SINO = 2
 
 _NAMES = {0: "CTR_OK", 1: "CTR_ERR_ARG", 2: "CTR_ERR_HIP", 3: "CTR_ERR_UNSUPPORTED",
           4: "CTR_ERR_NOMEM", 5: "CTR_ERR_NUMERIC"}

@@ -96,7 +96,7 @@ int build_tables(ctr_batch& B) {
       const T* vol = B.bf ? (const T*)c.xs.p : (const T*)c.x;
       T* tp = (T*)c.tp.p;
       FpBatchItem<T>& f = fp[(size_t)s * n + i];
-      f.a = fp_args<T>(P, vol, tp, s, S, c.prj, c.rowInv.f() + c.rowOff[s] * c.frame,
+      f.a = fp_args<T, T>(P, vol, tp, s, S, c.prj, c.rowInv.f() + c.rowOff[s] * c.frame,
                        c.partials.d());
       f.begin = fb;
       f.upv = fp_blocks_per_view(P, B.colw);
@@ -260,6 +260,8 @@ int ctr_batch_create(ctr_plan* const* plans, int nscan, float* const* d_x,
     if (!plans[i] || !plans[i]->impl || !d_x[i] || !d_prj[i])
       return fail(CTR_ERR_ARG, "NULL plan or buffer in the batch");
     Plan* P = plans[i]->impl;
+    if (P->storage == CTR_STORAGE_BF16_SINO)
+      return fail(CTR_ERR_UNSUPPORTED, "batched OS-SART runs with F32 or BF16 storage");
     if (i == 0) {
       B->device = P->device;
       B->bf = P->storage == CTR_STORAGE_BF16;

@@ -59,13 +59,13 @@ struct HostGeo {
   double ocx, ocy, ocz, dx, dz, S2D, bz0, bzN;
 };
 
-// Operands of one forward-projection launch over one scan (T: storage type of vol and
-// out, float or uint16_t holding BF16).  b / invRow / partials are set for the fused
+// Operands of one forward-projection launch over one scan (TV / TO: storage types of vol
+// and out, float or uint16_t holding BF16).  b / invRow / partials are set for the fused
 // OS-SART residual and NULL otherwise.
-template <typename T>
+template <typename TV, typename TO = TV>
 struct FpArgs {
-  const T* vol;
-  T* out;
+  const TV* vol;
+  TO* out;
   const ViewP* views;
   const float2* bnd;
   const float4* cols;
@@ -139,7 +139,8 @@ struct Plan {
   HostGeo hg;
   // projector method: CTR_METHOD_DIRECT or CTR_METHOD_SAT
   int method = CTR_METHOD_DIRECT;
-  // storage of the solvers' projector operands: CTR_STORAGE_F32 or CTR_STORAGE_BF16
+  // storage of the solvers' projector operands: CTR_STORAGE_F32, CTR_STORAGE_BF16 or
+  // CTR_STORAGE_BF16_SINO
   int storage = CTR_STORAGE_F32;
   // scratch
   float* sat_vol = nullptr;         // volume integral images (hi then lo planes)
@@ -166,8 +167,8 @@ int fail(int code, const std::string& msg);
 // kernel launchers (kernels.hip).  T is the storage type of the projector's input and
 // output arrays: float, or uint16_t holding BF16 (FP32 accumulation either way).
 // shadow: when not NULL, an UPDATE launch also writes the updated volume as BF16.
-template <typename T>
-int launch_fp(const Plan& P, const T* vol, T* out, int vb, int vs, int vc, hipStream_t st,
+template <typename TV, typename TO>
+int launch_fp(const Plan& P, const TV* vol, TO* out, int vb, int vs, int vc, hipStream_t st,
               const float* b, const float* invRow, double* partials);
 template <typename T>
 int launch_bp(const Plan& P, const T* prj, float* vol, int vb, int vs, int vc, int kind,
@@ -175,9 +176,9 @@ int launch_bp(const Plan& P, const T* prj, float* vol, int vb, int vs, int vc, i
 template <typename T>
 int launch_scale_matched(const Plan& P, const T* in, float* out, int vb, int vs, int vc,
                          hipStream_t st);
-template <typename T>
-FpArgs<T> fp_args(const Plan& P, const T* vol, T* out, int vb, int vs, const float* b,
-                  const float* invRow, double* partials);
+template <typename TV, typename TO>
+FpArgs<TV, TO> fp_args(const Plan& P, const TV* vol, TO* out, int vb, int vs, const float* b,
+                       const float* invRow, double* partials);
 template <typename T>
 BpArgs<T> bp_args(const Plan& P, const T* prj, float* vol, int vb, int vs, int vc,
                   const float* invCol, uint16_t* shadow);

@@ -29,13 +29,24 @@ __device__ __forceinline__ uint16_t to_bf16(float v) {
 __device__ __forceinline__ void st(uint16_t* p, size_t i, float v) { p[i] = to_bf16(v); }
 
 // BF16 with a fixed per-element dither, for the solvers' shadow of the iterate: the 16
-// dropped bits are raised by r(i) = frac(i * golden ratio) * 2^16 before truncation, so
-// element i rounds up with probability (dropped bits) / 2^16 over a region.  Round to
-// nearest gives every voxel of a flat region the same error, which a projection sums
-// (a data-residual floor near the BF16 precision); dithered errors average out along a
-// ray.  |error| < 1 BF16 ulp.
+// dropped bits are raised by r(i) in [0, 2^16) before truncation, so element i rounds up
+// with probability (dropped bits) / 2^16 over a region.  Round to nearest gives every
+// voxel of a flat region the same error, which a projection sums; the dithered errors
+// average out along a ray.  r(i) is the upper half of a well-mixed 32-bit hash of the
+// element index (the "lowbias32" integer finalizer published by C. Wellons): a linear
+// sequence in the index would leave the errors correlated along some ray directions.
+// |error| < 1 BF16 ulp.
+__device__ __forceinline__ uint32_t mix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x7feb352du;
+  h ^= h >> 15;
+  h *= 0x846ca68bu;
+  h ^= h >> 16;
+  return h;
+}
+
 __device__ __forceinline__ uint16_t to_bf16_dither(float v, size_t i) {
-  const uint32_t r = ((uint32_t)i * 2654435769u) >> 16;
+  const uint32_t r = mix32((uint32_t)i) >> 16;
   return (uint16_t)((__float_as_uint(v) + r) >> 16);
 }
 

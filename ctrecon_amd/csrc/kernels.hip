@@ -63,8 +63,8 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* slot) {
 // T: storage type of vol and out (float, or uint16_t for BF16); b and invRow are FP32
 // and the ray sum is accumulated in FP32.
 // ---------------------------------------------------------------------------
-template <bool FUSED, bool COLW, typename T>
-__device__ __forceinline__ void fp_body(const FpArgs<T>& A, int k, int ublk, double* slot) {
+template <bool FUSED, bool COLW, typename TV, typename TO>
+__device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk, double* slot) {
   const Dims& d = A.d;
   const int p = A.vb + k * A.vs;
   const int nray = d.DNU * d.DNV;
@@ -90,7 +90,7 @@ __device__ __forceinline__ void fp_body(const FpArgs<T>& A, int k, int ublk, dou
     const float ZNh = (float)d.ZN - 0.5f;
     const int ZN1 = d.ZN - 1;
     const float2* me = A.mext + (vp.xmarch ? 0 : d.XN);
-    const Buf<T> V(A.vol, (uint32_t)(d.XN * d.YN * d.ZN));
+    const Buf<TV> V(A.vol, (uint32_t)(d.XN * d.YN * d.ZN));
     const float* maskf = A.maskf;
     float acc = 0.f;
     // one slice: footprint [clo, chi] x [zl, zh] over the cells j0 .. j0 + nj - 1
@@ -206,12 +206,12 @@ __device__ __forceinline__ void fp_body(const FpArgs<T>& A, int k, int ublk, dou
   if constexpr (FUSED) block_sum2(res, bsq, slot);
 }
 
-template <bool FUSED, bool COLW, typename T>
-__global__ __launch_bounds__(kBlock) void fp_kernel(FpArgs<T> A) {
+template <bool FUSED, bool COLW, typename TV, typename TO>
+__global__ __launch_bounds__(kBlock) void fp_kernel(FpArgs<TV, TO> A) {
   const int hw = blockIdx.y * gridDim.x + blockIdx.x;
   const int lb = xcd_logical_block(hw, gridDim.x * gridDim.y);
   const int k = lb / gridDim.x;
-  fp_body<FUSED, COLW, T>(A, k, lb - k * gridDim.x, A.partials + 2 * hw);
+  fp_body<FUSED, COLW, TV, TO>(A, k, lb - k * gridDim.x, A.partials + 2 * hw);
 }
 
 // Batched persistent forward projection: each workgroup takes one contiguous range of
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(kBlock) void fp_batch_kernel(const FpBatchItem<T>* 
     const FpBatchItem<T>& B = D[s];
     const int local = item - B.begin;
     const int k = local / B.upv;
-    fp_body<FUSED, COLW, T>(B.a, k, local - k * B.upv, B.a.partials + 2 * local);
+    fp_body<FUSED, COLW, T, T>(B.a, k, local - k * B.upv, B.a.partials + 2 * local);
   }
 }
 
@@ -706,11 +706,11 @@ int fp_partial_blocks(const Plan& P, int vc) {
   return ((P.d.DNU * P.d.DNV + kBlock - 1) / kBlock) * vc;
 }
 
-template <typename T>
-FpArgs<T> fp_args(const Plan& P, const T* vol, T* out, int vb, int vs, const float* b,
-                  const float* invRow, double* partials) {
-  return FpArgs<T>{vol,  out,     P.views, P.bnd, P.cols, P.Bz,   P.h2, P.invdB, P.maskf,
-                   P.mext, P.d, vb,    vs,    b,      invRow, partials};
+template <typename TV, typename TO>
+FpArgs<TV, TO> fp_args(const Plan& P, const TV* vol, TO* out, int vb, int vs, const float* b,
+                       const float* invRow, double* partials) {
+  return FpArgs<TV, TO>{vol,    out, P.views, P.bnd, P.cols, P.Bz,   P.h2,    P.invdB,
+                        P.maskf, P.mext, P.d,  vb,    vs,     b,      invRow, partials};
 }
 
 template <typename T>
@@ -724,20 +724,20 @@ int fp_blocks_per_view(const Plan& P, bool colw) {
   return colw ? (P.d.DNU + kWaves - 1) / kWaves : (P.d.DNU * P.d.DNV + kBlock - 1) / kBlock;
 }
 
-template <typename T>
-int launch_fp(const Plan& P, const T* vol, T* out, int vb, int vs, int vc, hipStream_t st,
+template <typename TV, typename TO>
+int launch_fp(const Plan& P, const TV* vol, TO* out, int vb, int vs, int vc, hipStream_t st,
               const float* b, const float* invRow, double* partials) {
   if (vc <= 0) return CTR_OK;
   // with DNV == 64 both mappings have DNU * 64 / 256 blocks per view
   dim3 grid((P.d.DNU * P.d.DNV + kBlock - 1) / kBlock, vc);
   const bool colw = P.d.DNV == 64;
-  const FpArgs<T> A = fp_args<T>(P, vol, out, vb, vs, b, invRow, partials);
+  const FpArgs<TV, TO> A = fp_args<TV, TO>(P, vol, out, vb, vs, b, invRow, partials);
   if (b) {
-    if (colw) hipLaunchKernelGGL((fp_kernel<true, true, T>), grid, dim3(kBlock), 0, st, A);
-    else hipLaunchKernelGGL((fp_kernel<true, false, T>), grid, dim3(kBlock), 0, st, A);
+    if (colw) hipLaunchKernelGGL((fp_kernel<true, true, TV, TO>), grid, dim3(kBlock), 0, st, A);
+    else hipLaunchKernelGGL((fp_kernel<true, false, TV, TO>), grid, dim3(kBlock), 0, st, A);
   } else {
-    if (colw) hipLaunchKernelGGL((fp_kernel<false, true, T>), grid, dim3(kBlock), 0, st, A);
-    else hipLaunchKernelGGL((fp_kernel<false, false, T>), grid, dim3(kBlock), 0, st, A);
+    if (colw) hipLaunchKernelGGL((fp_kernel<false, true, TV, TO>), grid, dim3(kBlock), 0, st, A);
+    else hipLaunchKernelGGL((fp_kernel<false, false, TV, TO>), grid, dim3(kBlock), 0, st, A);
   }
   return check_launch("fp_kernel");
 }
@@ -844,18 +844,22 @@ int launch_scale_matched(const Plan& P, const T* in, float* out, int vb, int vs,
   return check_launch("scale_matched_kernel");
 }
 
-template int launch_fp<float>(const Plan&, const float*, float*, int, int, int, hipStream_t,
-                              const float*, const float*, double*);
-template int launch_fp<uint16_t>(const Plan&, const uint16_t*, uint16_t*, int, int, int,
-                                 hipStream_t, const float*, const float*, double*);
+template int launch_fp<float, float>(const Plan&, const float*, float*, int, int, int,
+                                     hipStream_t, const float*, const float*, double*);
+template int launch_fp<uint16_t, uint16_t>(const Plan&, const uint16_t*, uint16_t*, int, int,
+                                           int, hipStream_t, const float*, const float*,
+                                           double*);
+template int launch_fp<float, uint16_t>(const Plan&, const float*, uint16_t*, int, int, int,
+                                        hipStream_t, const float*, const float*, double*);
 template int launch_bp<float>(const Plan&, const float*, float*, int, int, int, int, hipStream_t,
                               const float*, int, uint16_t*);
 template int launch_bp<uint16_t>(const Plan&, const uint16_t*, float*, int, int, int, int,
                                  hipStream_t, const float*, int, uint16_t*);
-template FpArgs<float> fp_args<float>(const Plan&, const float*, float*, int, int, const float*,
-                                      const float*, double*);
-template FpArgs<uint16_t> fp_args<uint16_t>(const Plan&, const uint16_t*, uint16_t*, int, int,
-                                            const float*, const float*, double*);
+template FpArgs<float> fp_args<float, float>(const Plan&, const float*, float*, int, int,
+                                             const float*, const float*, double*);
+template FpArgs<uint16_t> fp_args<uint16_t, uint16_t>(const Plan&, const uint16_t*, uint16_t*,
+                                                      int, int, const float*, const float*,
+                                                      double*);
 template BpArgs<float> bp_args<float>(const Plan&, const float*, float*, int, int, int,
                                       const float*, uint16_t*);
 template BpArgs<uint16_t> bp_args<uint16_t>(const Plan&, const uint16_t*, float*, int, int, int,

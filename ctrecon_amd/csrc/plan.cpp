@@ -318,17 +318,17 @@ static int ensure_buf(float** p, size_t* have, size_t need, const char* what) {
 
 // Forward projection with the plan's method (fused residual when b != NULL); T is the
 // storage type of vol and out.
-template <typename T>
-static int fp_any(Plan& P, const T* vol, T* out, int vb, int vs, int vc, hipStream_t st,
+template <typename TV, typename TO = TV>
+static int fp_any(Plan& P, const TV* vol, TO* out, int vb, int vs, int vc, hipStream_t st,
                   const float* b, const float* invRow, double* partials) {
   if (P.method == CTR_METHOD_SAT) {
     int rc = ensure_buf(&P.sat_vol, &P.sat_vol_n, 2 * sat_volume_floats(P.d),
                         "volume integral images");
     if (rc) return rc;
-    if ((rc = launch_volume_sat<T>(P, vol, P.sat_vol, st))) return rc;
-    return launch_fp_sat<T>(P, P.sat_vol, out, vb, vs, vc, st, b, invRow, partials);
+    if ((rc = launch_volume_sat<TV>(P, vol, P.sat_vol, st))) return rc;
+    return launch_fp_sat<TO>(P, P.sat_vol, out, vb, vs, vc, st, b, invRow, partials);
   }
-  return launch_fp<T>(P, vol, out, vb, vs, vc, st, b, invRow, partials);
+  return launch_fp<TV, TO>(P, vol, out, vb, vs, vc, st, b, invRow, partials);
 }
 
 // Back projection (T: storage type of prj; vol FP32); shadow as in launch_bp.
@@ -394,7 +394,8 @@ struct ctr_sart {
   Plan* P = nullptr;
   const float* prj = nullptr;
   int osNum = 1, kind = CTR_BP_REFERENCE, maxV = 0;
-  bool bf = false;
+  bool bf = false;      // BF16 shadow of the iterate read by the forward projector
+  bool sino16 = false;  // BF16 residual sinogram between the projectors
   double setup_ms = 0.0;
   DBuf rowInv, colInv, tp, xs, sums;
   std::vector<size_t> rowOff;
@@ -514,8 +515,10 @@ int ctr_plan_bp(ctr_plan* plan, const float* d_prj, float* d_vol, int view_begin
 int ctr_plan_set_storage(ctr_plan* plan, int storage) {
   Plan* P = get(plan);
   if (!P) return fail(CTR_ERR_ARG, "NULL plan");
-  if (storage != CTR_STORAGE_F32 && storage != CTR_STORAGE_BF16)
-    return fail(CTR_ERR_ARG, "storage must be CTR_STORAGE_F32 or CTR_STORAGE_BF16");
+  if (storage != CTR_STORAGE_F32 && storage != CTR_STORAGE_BF16 &&
+      storage != CTR_STORAGE_BF16_SINO)
+    return fail(CTR_ERR_ARG,
+                "storage must be CTR_STORAGE_F32, CTR_STORAGE_BF16 or CTR_STORAGE_BF16_SINO");
   P->storage = storage;
   return CTR_OK;
 }
@@ -624,13 +627,15 @@ int ctr_sart_create(ctr_plan* plan, const float* d_prj, int osNum, int bp_kind, 
   S->osNum = osNum;
   S->kind = bp_kind;
   S->bf = P->storage == CTR_STORAGE_BF16;
+  S->sino16 = P->storage != CTR_STORAGE_F32;
   const Dims& d = P->d;
   const size_t NV = (size_t)d.XN * d.YN * d.ZN, frame = (size_t)d.DNU * d.DNV;
   S->maxV = (d.PN + osNum - 1) / osNum;
   int rc;
   if ((rc = S->rowInv.alloc(frame * d.PN * sizeof(float)))) return rc;  // subset-compact order
   if ((rc = S->colInv.alloc(NV * osNum * sizeof(float)))) return rc;
-  if ((rc = S->tp.alloc(frame * S->maxV * (S->bf ? sizeof(uint16_t) : sizeof(float))))) return rc;
+  if ((rc = S->tp.alloc(frame * S->maxV * (S->sino16 ? sizeof(uint16_t) : sizeof(float)))))
+    return rc;
   if (S->bf && (rc = S->xs.alloc(NV * sizeof(uint16_t)))) return rc;
   if ((rc = S->sums.alloc(2 * sizeof(double) * osNum))) return rc;
   if ((rc = ensure_partials(*P, 2 * (size_t)fp_partial_blocks(*P, S->maxV) + 2))) return rc;
@@ -705,9 +710,14 @@ int ctr_sart_sweeps(ctr_sart* S, float* d_x, int maxIter, double stop_resid, dou
       if (stats) CTR_HIP(hipEventRecord(kev[4 * s + 0], st));
       // tp = (prj - A_s x) ./ RowSum (os_sart.m:24,30-31)
       const float* rI = S->rowInv.f() + S->rowOff[s] * frame;
-      rc = S->bf ? fp_any<uint16_t>(*P, S->xs.h(), S->tp.h(), s, osNum, vc, st, S->prj, rI,
-                                    P->partials)
-                 : fp_any<float>(*P, d_x, S->tp.f(), s, osNum, vc, st, S->prj, rI, P->partials);
+      if (S->bf)
+        rc = fp_any<uint16_t, uint16_t>(*P, S->xs.h(), S->tp.h(), s, osNum, vc, st, S->prj, rI,
+                                        P->partials);
+      else if (S->sino16)
+        rc = fp_any<float, uint16_t>(*P, d_x, S->tp.h(), s, osNum, vc, st, S->prj, rI,
+                                     P->partials);
+      else
+        rc = fp_any<float, float>(*P, d_x, S->tp.f(), s, osNum, vc, st, S->prj, rI, P->partials);
       if (rc) return rc;
       if (stats) CTR_HIP(hipEventRecord(kev[4 * s + 1], st));
       if ((rc = launch_reduce_pairs(P->partials, fp_partial_blocks(*P, vc), S->sums.d() + 2 * s,
@@ -716,9 +726,9 @@ int ctr_sart_sweeps(ctr_sart* S, float* d_x, int maxIter, double stop_resid, dou
       if (stats) CTR_HIP(hipEventRecord(kev[4 * s + 2], st));
       // x += B_s(tp) ./ ColSum (os_sart.m:32,37-40)
       const float* cI = S->colInv.f() + NV * s;
-      rc = S->bf ? bp_any<uint16_t>(*P, S->tp.h(), d_x, s, osNum, vc, S->kind, st, cI, 1,
-                                    S->xs.h())
-                 : bp_any<float>(*P, S->tp.f(), d_x, s, osNum, vc, S->kind, st, cI, 1);
+      rc = S->sino16 ? bp_any<uint16_t>(*P, S->tp.h(), d_x, s, osNum, vc, S->kind, st, cI, 1,
+                                        S->bf ? S->xs.h() : nullptr)
+                     : bp_any<float>(*P, S->tp.f(), d_x, s, osNum, vc, S->kind, st, cI, 1);
       if (rc) return rc;
       if (stats) CTR_HIP(hipEventRecord(kev[4 * s + 3], st));
     }
@@ -799,11 +809,12 @@ int ctr_conjgrad(ctr_plan* plan, float* d_x, const float* d_prj, int iterNum, in
   // BF16 storage: A reads a BF16 copy of its input and passes a BF16 sinogram between
   // the projectors; the CG vectors stay FP32 and the dots / scalars FP64
   const bool bf = P->storage == CTR_STORAGE_BF16;
+  const bool sino16 = P->storage != CTR_STORAGE_F32;
   DBuf r, pv, t, sino, dots, in16;
   if ((rc = r.alloc(NV * sizeof(float)))) return rc;
   if ((rc = pv.alloc(NV * sizeof(float)))) return rc;
   if ((rc = t.alloc(NV * sizeof(float)))) return rc;
-  if ((rc = sino.alloc(NS * (bf ? sizeof(uint16_t) : sizeof(float))))) return rc;
+  if ((rc = sino.alloc(NS * (sino16 ? sizeof(uint16_t) : sizeof(float))))) return rc;
   if (bf && (rc = in16.alloc(NV * sizeof(uint16_t)))) return rc;
   if ((rc = dots.alloc(2 * sizeof(double)))) return rc;
   const int nparts = 1024;
@@ -818,7 +829,13 @@ int ctr_conjgrad(ctr_plan* plan, float* d_x, const float* d_prj, int iterNum, in
         return e;
       return bp_any<uint16_t>(*P, sino.h(), out, 0, 1, d.PN, bp_kind, st, nullptr, 0);
     }
-    int e = fp_any<float>(*P, in, sino.f(), 0, 1, d.PN, st, nullptr, nullptr, nullptr);
+    if (sino16) {
+      int e = fp_any<float, uint16_t>(*P, in, sino.h(), 0, 1, d.PN, st, nullptr, nullptr,
+                                      nullptr);
+      if (e) return e;
+      return bp_any<uint16_t>(*P, sino.h(), out, 0, 1, d.PN, bp_kind, st, nullptr, 0);
+    }
+    int e = fp_any<float, float>(*P, in, sino.f(), 0, 1, d.PN, st, nullptr, nullptr, nullptr);
     if (e) return e;
     return bp_any<float>(*P, sino.f(), out, 0, 1, d.PN, bp_kind, st, nullptr, 0);
   };

@@ -162,9 +162,11 @@ class Plan:
         self.method = method
 
     def set_storage(self, storage):
-        """Storage of the OS-SART / CG drivers' projector operands: 'f32' (default) or
-        'bf16' (FP32 master iterate, BF16 shadow and BF16 residual sinogram)."""
-        code = {"f32": _lib.CTR_STORAGE_F32, "bf16": _lib.CTR_STORAGE_BF16}[storage]
+        """Storage of the OS-SART / CG drivers' projector operands: 'f32' (default),
+        'bf16' (FP32 master iterate, BF16 shadow and BF16 residual sinogram) or
+        'bf16_sino' (BF16 residual sinogram only)."""
+        code = {"f32": _lib.CTR_STORAGE_F32, "bf16": _lib.CTR_STORAGE_BF16,
+                "bf16_sino": _lib.CTR_STORAGE_BF16_SINO}[storage]
         check(lib().ctr_plan_set_storage(self.handle, code))
         self.storage = storage
 

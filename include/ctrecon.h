@@ -47,7 +47,8 @@ extern "C" {
  * BF16 arrays are unsigned short bit patterns: the upper half of the FP32 value,
  * rounded to nearest even.  Every sum is accumulated in FP32 or wider. */
 #define CTR_STORAGE_F32 0
-#define CTR_STORAGE_BF16 1
+#define CTR_STORAGE_BF16 1      /* volume and sinogram operands BF16 */
+#define CTR_STORAGE_BF16_SINO 2 /* sinogram operands BF16, the iterate read in FP32 */
 
 /*
  * Forward projection, host buffers in and out.
@@ -110,10 +111,11 @@ int ctr_plan_set_method(ctr_plan* plan, int method);
 
 /* Storage the OS-SART and CG drivers use for their projector operands (build
  * addition, no reference counterpart).  CTR_STORAGE_BF16: the iterate keeps its FP32
- * master copy (d_x) and a BF16 shadow that the forward projector reads; the sinogram
- * passed from the forward to the back projector is BF16; the measured sinogram d_prj,
- * the row / column sums, the update and every reduction stay FP32 / FP64.
- * ctr_residual always projects in FP32. */
+ * master copy (d_x) and a dithered BF16 shadow that the forward projector reads; the
+ * sinogram passed from the forward to the back projector is BF16.
+ * CTR_STORAGE_BF16_SINO: only that sinogram is BF16 (the forward projector reads the
+ * FP32 iterate).  In every mode the measured sinogram d_prj, the row / column sums, the
+ * update and every reduction stay FP32 / FP64.  ctr_residual always projects in FP32. */
 int ctr_plan_set_storage(ctr_plan* plan, int storage);
 
 /* Forward-project device volume d_vol into device sinogram d_prj for the views
