@@ -210,22 +210,28 @@ def _phantom_job(nxy, nz, seed):
     return training_phantom(g, seed)
 
 
-def compare(plan: Plan, net: UNet, vol, target=1e-3, osNum=20, max_sweeps=400, kind="ref"):
-    """Plain OS-SART vs the warm start on b = A vol with the plan's storage: sweeps and
-    wall times to the residual target."""
+def compare(plan: Plan, net: UNet, vol, target=1e-3, osNum=20, max_sweeps=400, kind="ref",
+            plain_storage=None, warm_storage=None):
+    """Plain OS-SART vs the warm start on b = A vol (storage modes default to the
+    plan's): sweeps and wall times to the residual target."""
     g = plan.g
+    keep = getattr(plan, "storage", "f32")
     d_b = DeviceArray(g.prj_shape)
+    plan.set_storage("f32")
     plan.fp(DeviceArray.from_host(vol), d_b)
     dx = DeviceArray(g.vol_shape)
     dx.zero()
+    plan.set_storage(plain_storage or keep)
     plan.synchronize()
     t0 = time.perf_counter()
     S = Sart(plan, d_b, osNum, kind)
     r_plain = S.sweeps(dx, max_sweeps, target)[0]
     plan.synchronize()
     t_plain = time.perf_counter() - t0
-    info = warm_start_os_sart(plan, net, d_b, dx, osNum, target, max_sweeps, kind)
     S.close()
+    plan.set_storage(warm_storage or keep)
+    info = warm_start_os_sart(plan, net, d_b, dx, osNum, target, max_sweeps, kind)
+    plan.set_storage(keep)
     return {"plain_sweeps": len(r_plain), "plain_s": t_plain, "plain_final": float(r_plain[-1]),
             "warm_sweeps": info["sweeps"], "warm_s": info["total_s"],
             "warm_final": float(info["residuals"][-1]), "fallback": info["fallback"],
@@ -236,10 +242,11 @@ def compare(plan: Plan, net: UNet, vol, target=1e-3, osNum=20, max_sweeps=400, k
 def _eval(a):
     g = G.dd3_geometry(G.config_training(a.nxy, a.nz), mask=G.fov_mask(a.nxy))
     plan = Plan(g, 0)
-    plan.set_storage("bf16")
     net = UNet.from_file(a.weights)
     for seed in a.seeds:
-        r = compare(plan, net, training_phantom(g, seed), a.target, a.subsets, a.max_sweeps)
+        vol = PH.ellipsoid_mixture(g, seed=seed) if a.mixture else training_phantom(g, seed)
+        r = compare(plan, net, vol, a.target, a.subsets, a.max_sweeps,
+                    plain_storage=a.plain_storage, warm_storage=a.warm_storage)
         print(f"seed {seed}: " + ", ".join(f"{k} {v:.4g}" if isinstance(v, float) else f"{k} {v}"
                                            for k, v in r.items()), flush=True)
 
@@ -276,6 +283,10 @@ def main(argv=None):
     ev.add_argument("--nxy", type=int, default=256)
     ev.add_argument("--nz", type=int, default=128)
     ev.add_argument("--max-sweeps", type=int, default=400)
+    ev.add_argument("--mixture", action="store_true",
+                    help="plain 40-ellipsoid mixtures (the benchmark's phantom recipe)")
+    ev.add_argument("--plain-storage", default="f32")
+    ev.add_argument("--warm-storage", default="bf16")
     a = ap.parse_args(argv)
     if a.cmd == "eval":
         return _eval(a)
