@@ -76,8 +76,9 @@ class CtrSartStats(ctypes.Structure):
 
 class CtrBatchInfo(ctypes.Structure):
     _fields_ = [
-        ("fp_colw", ctypes.c_int), ("fp_grid", ctypes.c_int), ("bp_zg", ctypes.c_int),
-        ("bp_grid", ctypes.c_int), ("setup_ms", ctypes.c_double), ("tune_ms", ctypes.c_double),
+        ("fp_colw", ctypes.c_int), ("fp_grid", ctypes.c_int), ("fp_strided", ctypes.c_int),
+        ("bp_zg", ctypes.c_int), ("bp_grid", ctypes.c_int), ("bp_strided", ctypes.c_int),
+        ("setup_ms", ctypes.c_double), ("tune_ms", ctypes.c_double),
         ("capture_ms", ctypes.c_double), ("fp_items", ctypes.c_longlong),
         ("bp_items", ctypes.c_longlong),
     ]
@@ -130,7 +131,7 @@ def lib():
         L.ctr_batch_create.argtypes = [p, i, p, p, i, i, ctypes.POINTER(p)]
         L.ctr_batch_sweeps.argtypes = [p, i, d, p, p, ctypes.POINTER(i)]
         L.ctr_batch_set_graph.argtypes = [p, i]
-        L.ctr_batch_set_tuning.argtypes = [p, i, i, i, i]
+        L.ctr_batch_set_tuning.argtypes = [p, i, i, i, i, i, i]
         L.ctr_batch_get_info.argtypes = [p, ctypes.POINTER(CtrBatchInfo)]
         L.ctr_batch_destroy.argtypes = [p]
         ll = ctypes.c_longlong

@@ -39,9 +39,12 @@ class Batch:
     def set_graph(self, enable: bool):
         check(lib().ctr_batch_set_graph(self.handle, int(bool(enable))))
 
-    def set_tuning(self, fp_colw: bool, fp_grid: int, bp_zg: int, bp_grid: int):
-        check(lib().ctr_batch_set_tuning(self.handle, int(bool(fp_colw)), fp_grid, bp_zg,
-                                         bp_grid))
+    def set_tuning(self, fp_colw: bool, fp_grid: int, bp_zg: int, bp_grid: int,
+                   fp_strided=False, bp_strided=False):
+        """Launch shapes; a grid of 0 runs one workgroup per work item."""
+        check(lib().ctr_batch_set_tuning(self.handle, int(bool(fp_colw)), fp_grid,
+                                         int(bool(fp_strided)), bp_zg, bp_grid,
+                                         int(bool(bp_strided))))
 
     def info(self) -> dict:
         i = CtrBatchInfo()

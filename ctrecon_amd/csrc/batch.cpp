@@ -63,7 +63,8 @@ struct ctr_batch {
   Mem fpItems, bpItems, tuneBpItems, redItems;
   std::vector<int> fpTotal, bpTotal;
   bool colw = false;
-  int fpGrid = 0, zg = 1, bpGrid = 0;
+  int fpGrid = 0, zg = 1, bpGrid = 0;  // grids: persistent workgroups, <= 0 one per item
+  int fpStrided = 0, bpStrided = 0;
   hipStream_t st = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -137,14 +138,14 @@ int build(ctr_batch& B) { return B.bf ? build_tables<uint16_t>(B) : build_tables
 template <typename T>
 int step_fp(ctr_batch& B, int s) {
   return launch_fp_batch<T>((const FpBatchItem<T>*)B.fpItems.p + (size_t)s * B.n, B.n,
-                            B.fpTotal[s], B.colw, B.fpGrid, B.st);
+                            B.fpTotal[s], B.colw, B.fpGrid, B.fpStrided, B.st);
 }
 
 template <typename T>
 int step_bp(ctr_batch& B, int s, bool tuning) {
   const Mem& m = tuning ? B.tuneBpItems : B.bpItems;
   return launch_bp_batch<T>((const BpBatchItem<T>*)m.p + (size_t)s * B.n, B.n, B.bpTotal[s],
-                            B.zg, B.bpGrid, B.st);
+                            B.zg, B.bpGrid, B.bpStrided, B.st);
 }
 
 int sweep_launches(ctr_batch& B) {
@@ -193,51 +194,59 @@ int autotune(ctr_batch& B) {
     all64 = all64 && c->P->d.DNV == 64;
     maxZN = std::max(maxZN, c->P->d.ZN);
   }
-  const int mult[3] = {2, 4, 8};
+  // schedules: (grid, strided); grid 0 = one workgroup per item
+  const int sched[4][2] = {{4 * B.ncu, 0}, {4 * B.ncu, 1}, {8 * B.ncu, 1}, {0, 0}};
   float best = 1e30f;
   bool bcolw = false;
-  int bgrid = 0, rc;
+  int bgrid = 0, bstr = 0, rc;
   for (int cw = all64 ? 1 : 0; cw >= 0; --cw) {
     B.colw = cw;
     if ((rc = build(B))) return rc;
-    for (int m : mult) {
-      B.fpGrid = m * B.ncu;
+    for (const auto& sc : sched) {
+      B.fpGrid = sc[0];
+      B.fpStrided = sc[1];
       float ms = 0;
       auto f = [&]() { return B.bf ? step_fp<uint16_t>(B, 0) : step_fp<float>(B, 0); };
-      if ((rc = time_launches(B, f, 2, &ms))) return rc;
+      if ((rc = time_launches(B, f, 1, &ms))) return rc;
       if (ms < best) {
         best = ms;
         bcolw = cw;
         bgrid = B.fpGrid;
+        bstr = B.fpStrided;
       }
     }
   }
   B.colw = bcolw;
   B.fpGrid = bgrid;
+  B.fpStrided = bstr;
   best = 1e30f;
   int bzg = 1;
   bgrid = 0;
+  bstr = 0;
   const int zgs[3] = {1, 2, 4};
   for (int zg : zgs) {
     if (zg > 1 && 64 * (zg / 2) >= maxZN) break;  // more groups than any column needs
     B.zg = zg;
     if ((rc = build(B))) return rc;
-    for (int m : mult) {
-      B.bpGrid = m * B.ncu;
+    for (const auto& sc : sched) {
+      B.bpGrid = sc[0];
+      B.bpStrided = sc[1];
       float ms = 0;
       auto f = [&]() {
         return B.bf ? step_bp<uint16_t>(B, 0, true) : step_bp<float>(B, 0, true);
       };
-      if ((rc = time_launches(B, f, 2, &ms))) return rc;
+      if ((rc = time_launches(B, f, 1, &ms))) return rc;
       if (ms < best) {
         best = ms;
         bzg = zg;
         bgrid = B.bpGrid;
+        bstr = B.bpStrided;
       }
     }
   }
   B.zg = bzg;
   B.bpGrid = bgrid;
+  B.bpStrided = bstr;
   return build(B);
 }
 
@@ -349,9 +358,10 @@ int ctr_batch_set_graph(ctr_batch* B, int enable) {
   return CTR_OK;
 }
 
-int ctr_batch_set_tuning(ctr_batch* B, int fp_colw, int fp_grid, int bp_zg, int bp_grid) {
+int ctr_batch_set_tuning(ctr_batch* B, int fp_colw, int fp_grid, int fp_strided, int bp_zg,
+                         int bp_grid, int bp_strided) {
   if (!B) return fail(CTR_ERR_ARG, "NULL batch");
-  if (bp_zg < 1 || bp_zg > 4 || fp_grid < 1 || bp_grid < 1)
+  if (bp_zg < 1 || bp_zg > 4 || fp_grid < 0 || bp_grid < 0)
     return fail(CTR_ERR_ARG, "bad tuning");
   if (fp_colw)
     for (auto& c : B->scans)
@@ -359,8 +369,10 @@ int ctr_batch_set_tuning(ctr_batch* B, int fp_colw, int fp_grid, int bp_zg, int 
   CTR_HIP(hipSetDevice(B->device));
   B->colw = fp_colw != 0;
   B->fpGrid = fp_grid;
+  B->fpStrided = fp_strided != 0;
   B->zg = bp_zg;
   B->bpGrid = bp_grid;
+  B->bpStrided = bp_strided != 0;
   drop_graph(*B);
   return build(*B);
 }
@@ -369,8 +381,10 @@ int ctr_batch_get_info(const ctr_batch* B, ctr_batch_info* info) {
   if (!B || !info) return fail(CTR_ERR_ARG, "NULL argument");
   info->fp_colw = B->colw;
   info->fp_grid = B->fpGrid;
+  info->fp_strided = B->fpStrided;
   info->bp_zg = B->zg;
   info->bp_grid = B->bpGrid;
+  info->bp_strided = B->bpStrided;
   info->setup_ms = B->setup_ms;
   info->tune_ms = B->tune_ms;
   info->capture_ms = B->capture_ms;

@@ -183,13 +183,14 @@ template <typename T>
 BpArgs<T> bp_args(const Plan& P, const T* prj, float* vol, int vb, int vs, int vc,
                   const float* invCol, uint16_t* shadow);
 int fp_blocks_per_view(const Plan& P, bool colw);
-// batched persistent launches (grid = number of workgroups)
+// batched launches: grid = number of persistent workgroups (<= 0: one per work item),
+// strided = interleaved instead of contiguous item ranges
 template <typename T>
 int launch_fp_batch(const FpBatchItem<T>* items, int nscan, int total, bool colw, int grid,
-                    hipStream_t st);
+                    int strided, hipStream_t st);
 template <typename T>
 int launch_bp_batch(const BpBatchItem<T>* items, int nscan, int total, int zg, int grid,
-                    hipStream_t st);
+                    int strided, hipStream_t st);
 int launch_reduce_batch(const ReduceItem* items, int nscan, hipStream_t st);
 int launch_to_bf16(uint16_t* dst, const float* src, size_t n, hipStream_t st);
 // the solvers' dithered BF16 shadow of an iterate (to_bf16_dither)

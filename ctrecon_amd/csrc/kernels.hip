@@ -217,14 +217,34 @@ __global__ __launch_bounds__(kBlock) void fp_kernel(FpArgs<TV, TO> A) {
 // Batched persistent forward projection: each workgroup takes one contiguous range of
 // the flat work list (XCD-aware, as in fp_kernel), items of one scan being
 // consecutive (view-major, then column blocks).
+// Work assignment of the batched kernels: the workgroup's first item, the item step and
+// the end.  Contiguous (strided == 0): one range per workgroup, ranges ordered per XCD
+// (neighbouring items share an L2); strided: items lb, lb + G, ... (every workgroup gets
+// a sample of the whole list, which balances uneven item costs).  A grid of one
+// workgroup per item is the hardware-scheduled case of either.
+__device__ __forceinline__ void batch_range(int total, int strided, int& i0, int& step,
+                                            int& i1) {
+  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  if (strided) {
+    i0 = lb;
+    step = gridDim.x;
+    i1 = total;
+  } else {
+    const int per = (total + gridDim.x - 1) / gridDim.x;
+    i0 = lb * per;
+    step = 1;
+    i1 = min(total, i0 + per);
+  }
+}
+
 template <bool FUSED, bool COLW, typename T>
 __global__ __launch_bounds__(kBlock) void fp_batch_kernel(const FpBatchItem<T>* __restrict__ D,
-                                                          int nscan, int total) {
-  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-  const int per = (total + gridDim.x - 1) / gridDim.x;
-  const int i0 = lb * per, i1 = min(total, i0 + per);
+                                                          int nscan, int total, int strided) {
+  int i0, step, i1;
+  batch_range(total, strided, i0, step, i1);
   int s = 0;
-  for (int item = i0; item < i1; ++item) {
+  for (int item = i0; item < i1; item += step) {
+    if (strided) s = 0;
     while (s + 1 < nscan && D[s + 1].begin <= item) ++s;
     const FpBatchItem<T>& B = D[s];
     const int local = item - B.begin;
@@ -417,13 +437,13 @@ __global__ __launch_bounds__(kBlock) void bp_ref_kernel(BpArgs<T> A) {
 // block of 4 columns), contiguous ranges per workgroup as in fp_batch_kernel.
 template <bool UPDATE, int ZG, typename T>
 __global__ __launch_bounds__(kBlock) void bp_batch_kernel(const BpBatchItem<T>* __restrict__ D,
-                                                          int nscan, int total) {
-  const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-  const int per = (total + gridDim.x - 1) / gridDim.x;
-  const int i0 = lb * per, i1 = min(total, i0 + per);
+                                                          int nscan, int total, int strided) {
+  int i0, step, i1;
+  batch_range(total, strided, i0, step, i1);
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   int s = 0;
-  for (int item = i0; item < i1; ++item) {
+  for (int item = i0; item < i1; item += step) {
+    if (strided) s = 0;
     while (s + 1 < nscan && D[s + 1].begin <= item) ++s;
     const BpBatchItem<T>& B = D[s];
     const int local = item - B.begin;
@@ -796,28 +816,29 @@ int launch_bp(const Plan& P, const T* prj, float* vol, int vb, int vs, int vc, i
   return check_launch("bp_ref_kernel");
 }
 
+// grid <= 0: one workgroup per item (hardware scheduling)
 template <typename T>
 int launch_fp_batch(const FpBatchItem<T>* items, int nscan, int total, bool colw, int grid,
-                    hipStream_t st) {
+                    int strided, hipStream_t st) {
   if (total <= 0) return CTR_OK;
-  grid = std::max(1, std::min(grid, total));
+  grid = grid <= 0 ? total : std::max(1, std::min(grid, total));
   if (colw)
     hipLaunchKernelGGL((fp_batch_kernel<true, true, T>), dim3(grid), dim3(kBlock), 0, st, items,
-                       nscan, total);
+                       nscan, total, strided);
   else
     hipLaunchKernelGGL((fp_batch_kernel<true, false, T>), dim3(grid), dim3(kBlock), 0, st, items,
-                       nscan, total);
+                       nscan, total, strided);
   return check_launch("fp_batch_kernel");
 }
 
 template <typename T>
 int launch_bp_batch(const BpBatchItem<T>* items, int nscan, int total, int zg, int grid,
-                    hipStream_t st) {
+                    int strided, hipStream_t st) {
   if (total <= 0) return CTR_OK;
-  grid = std::max(1, std::min(grid, total));
+  grid = grid <= 0 ? total : std::max(1, std::min(grid, total));
 #define CTR_BPB(ZGV)                                                                       \
   hipLaunchKernelGGL((bp_batch_kernel<true, ZGV, T>), dim3(grid), dim3(kBlock), 0, st, items, \
-                     nscan, total)
+                     nscan, total, strided)
   switch (zg) {
     case 1: CTR_BPB(1); break;
     case 2: CTR_BPB(2); break;
@@ -864,11 +885,13 @@ template BpArgs<float> bp_args<float>(const Plan&, const float*, float*, int, in
                                       const float*, uint16_t*);
 template BpArgs<uint16_t> bp_args<uint16_t>(const Plan&, const uint16_t*, float*, int, int, int,
                                             const float*, uint16_t*);
-template int launch_fp_batch<float>(const FpBatchItem<float>*, int, int, bool, int, hipStream_t);
-template int launch_fp_batch<uint16_t>(const FpBatchItem<uint16_t>*, int, int, bool, int,
+template int launch_fp_batch<float>(const FpBatchItem<float>*, int, int, bool, int, int,
+                                    hipStream_t);
+template int launch_fp_batch<uint16_t>(const FpBatchItem<uint16_t>*, int, int, bool, int, int,
                                        hipStream_t);
-template int launch_bp_batch<float>(const BpBatchItem<float>*, int, int, int, int, hipStream_t);
-template int launch_bp_batch<uint16_t>(const BpBatchItem<uint16_t>*, int, int, int, int,
+template int launch_bp_batch<float>(const BpBatchItem<float>*, int, int, int, int, int,
+                                    hipStream_t);
+template int launch_bp_batch<uint16_t>(const BpBatchItem<uint16_t>*, int, int, int, int, int,
                                        hipStream_t);
 template int launch_scale_matched<float>(const Plan&, const float*, float*, int, int, int,
                                          hipStream_t);

@@ -207,16 +207,19 @@ int ctr_residual(ctr_plan* plan, const float* d_x, const float* d_prj, double* o
  * their iterates d_x[i] and measured sinograms d_prj[i] (kept by the batch until
  * destroyed).  ctr_batch_create computes every scan's row / column sums and tunes the
  * launch shapes on the batch (persistent grid sizes, forward work mapping, z groups per
- * lane of the back projector, chosen by timing); each subset step then runs as one
+ * lane of the back projector, contiguous / interleaved / one-per-item work assignment,
+ * chosen by timing); each subset step then runs as one
  * forward, one reduction and one back-projection launch over all scans, and a whole
  * sweep is captured once into a HIP graph and replayed.  Reference back projector
  * only. */
 typedef struct ctr_batch ctr_batch;
 typedef struct ctr_batch_info {
   int fp_colw;        /* forward mapping: 1 = one wave per detector column, 0 = per ray */
-  int fp_grid;        /* persistent workgroups of the forward launch */
+  int fp_grid;        /* persistent workgroups of the forward launch (0: one per item) */
+  int fp_strided;     /* 1: interleaved item ranges, 0: contiguous */
   int bp_zg;          /* z groups per lane of the back projector */
-  int bp_grid;        /* persistent workgroups of the back-projection launch */
+  int bp_grid;        /* persistent workgroups of the back-projection launch (0: one per item) */
+  int bp_strided;     /* 1: interleaved item ranges, 0: contiguous */
   double setup_ms;    /* row / column sums of all scans */
   double tune_ms;     /* autotuning (host wall time) */
   double capture_ms;  /* graph capture + instantiation (host wall time; 0 before) */
@@ -232,7 +235,8 @@ int ctr_batch_sweeps(ctr_batch* batch, int maxIter, double stop_resid, double* r
 /* Graph replay on (default) or plain launches. */
 int ctr_batch_set_graph(ctr_batch* batch, int enable);
 /* Override the tuned launch shapes (tests and experiments). */
-int ctr_batch_set_tuning(ctr_batch* batch, int fp_colw, int fp_grid, int bp_zg, int bp_grid);
+int ctr_batch_set_tuning(ctr_batch* batch, int fp_colw, int fp_grid, int fp_strided, int bp_zg,
+                         int bp_grid, int bp_strided);
 int ctr_batch_get_info(const ctr_batch* batch, ctr_batch_info* info);
 int ctr_batch_destroy(ctr_batch* batch);
 

@@ -125,11 +125,12 @@ def test_tuned_batch_matches_per_scan(gpu_plan_factory):
         assert e <= 1e-5
         assert np.allclose(res[:, i], r_ref, rtol=1e-5)
     # every tuning candidate computes the same iterates
-    for colw in (True, False):
-        for zg in (1, 2, 4):
-            for x in xs:
-                x.zero()
-            B.set_tuning(colw, 1024, zg, 2048)
-            B.sweeps(3)
-            for i, (x_ref, _) in enumerate(ref):
-                assert rel_l2(xs[i].get(), x_ref) <= 1e-5, (colw, zg, i)
+    for colw, zg, grid, strided in [(True, 1, 1024, False), (False, 2, 2048, True),
+                                    (True, 4, 0, False), (False, 1, 0, True),
+                                    (True, 2, 512, True)]:
+        for x in xs:
+            x.zero()
+        B.set_tuning(colw, grid, zg, grid, strided, strided)
+        B.sweeps(3)
+        for i, (x_ref, _) in enumerate(ref):
+            assert rel_l2(xs[i].get(), x_ref) <= 1e-5, (colw, zg, grid, strided, i)
