@@ -217,15 +217,31 @@ __global__ __launch_bounds__(kBlock) void fp_kernel(FpArgs<TV, TO> A) {
 // Batched persistent forward projection: each workgroup takes one contiguous range of
 // the flat work list (XCD-aware, as in fp_kernel), items of one scan being
 // consecutive (view-major, then column blocks).
-// Work assignment of the batched kernels: the workgroup's first item, the item step and
-// the end.  Contiguous (strided == 0): one range per workgroup, ranges ordered per XCD
-// (neighbouring items share an L2); strided: items lb, lb + G, ... (every workgroup gets
-// a sample of the whole list, which balances uneven item costs).  A grid of one
-// workgroup per item is the hardware-scheduled case of either.
-__device__ __forceinline__ void batch_range(int total, int strided, int& i0, int& step,
-                                            int& i1) {
+// Work assignment of the batched kernels (sched): the workgroup's first item, the item
+// step and the end.
+//  0 contiguous: one range per workgroup, ranges ordered per XCD (neighbouring items
+//    share an L2);
+//  1 strided: items lb, lb + G, ... (every workgroup samples the whole list, which
+//    balances uneven item costs);
+//  2 one workgroup per item (G == total): workgroups are dispatched in order, so the
+//    items of one scan run together on all XCDs, and within a scan's range the
+//    workgroup -> item map is the XCD-aware map of the single-scan kernels.
+template <typename Item>
+__device__ __forceinline__ void batch_range(const Item* __restrict__ D, int nscan, int total,
+                                            int sched, int& i0, int& step, int& i1) {
+  if (sched == 2) {
+    const int hw = blockIdx.x;
+    int s = 0;
+    while (s + 1 < nscan && D[s + 1].begin <= hw) ++s;
+    const int b0 = D[s].begin;
+    const int n = (s + 1 < nscan ? D[s + 1].begin : total) - b0;
+    i0 = b0 + xcd_logical_block(hw - b0, n);
+    step = 1;
+    i1 = i0 + 1;
+    return;
+  }
   const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
-  if (strided) {
+  if (sched == 1) {
     i0 = lb;
     step = gridDim.x;
     i1 = total;
@@ -239,12 +255,12 @@ __device__ __forceinline__ void batch_range(int total, int strided, int& i0, int
 
 template <bool FUSED, bool COLW, typename T>
 __global__ __launch_bounds__(kBlock) void fp_batch_kernel(const FpBatchItem<T>* __restrict__ D,
-                                                          int nscan, int total, int strided) {
+                                                          int nscan, int total, int sched) {
   int i0, step, i1;
-  batch_range(total, strided, i0, step, i1);
+  batch_range(D, nscan, total, sched, i0, step, i1);
   int s = 0;
   for (int item = i0; item < i1; item += step) {
-    if (strided) s = 0;
+    if (sched == 1) s = 0;
     while (s + 1 < nscan && D[s + 1].begin <= item) ++s;
     const FpBatchItem<T>& B = D[s];
     const int local = item - B.begin;
@@ -437,13 +453,13 @@ __global__ __launch_bounds__(kBlock) void bp_ref_kernel(BpArgs<T> A) {
 // block of 4 columns), contiguous ranges per workgroup as in fp_batch_kernel.
 template <bool UPDATE, int ZG, typename T>
 __global__ __launch_bounds__(kBlock) void bp_batch_kernel(const BpBatchItem<T>* __restrict__ D,
-                                                          int nscan, int total, int strided) {
+                                                          int nscan, int total, int sched) {
   int i0, step, i1;
-  batch_range(total, strided, i0, step, i1);
+  batch_range(D, nscan, total, sched, i0, step, i1);
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   int s = 0;
   for (int item = i0; item < i1; item += step) {
-    if (strided) s = 0;
+    if (sched == 1) s = 0;
     while (s + 1 < nscan && D[s + 1].begin <= item) ++s;
     const BpBatchItem<T>& B = D[s];
     const int local = item - B.begin;
@@ -816,18 +832,19 @@ int launch_bp(const Plan& P, const T* prj, float* vol, int vb, int vs, int vc, i
   return check_launch("bp_ref_kernel");
 }
 
-// grid <= 0: one workgroup per item (hardware scheduling)
+// grid <= 0: one workgroup per item (hardware scheduling, per-scan XCD map)
 template <typename T>
 int launch_fp_batch(const FpBatchItem<T>* items, int nscan, int total, bool colw, int grid,
                     int strided, hipStream_t st) {
   if (total <= 0) return CTR_OK;
+  const int sched = grid <= 0 ? 2 : (strided ? 1 : 0);
   grid = grid <= 0 ? total : std::max(1, std::min(grid, total));
   if (colw)
     hipLaunchKernelGGL((fp_batch_kernel<true, true, T>), dim3(grid), dim3(kBlock), 0, st, items,
-                       nscan, total, strided);
+                       nscan, total, sched);
   else
     hipLaunchKernelGGL((fp_batch_kernel<true, false, T>), dim3(grid), dim3(kBlock), 0, st, items,
-                       nscan, total, strided);
+                       nscan, total, sched);
   return check_launch("fp_batch_kernel");
 }
 
@@ -835,10 +852,11 @@ template <typename T>
 int launch_bp_batch(const BpBatchItem<T>* items, int nscan, int total, int zg, int grid,
                     int strided, hipStream_t st) {
   if (total <= 0) return CTR_OK;
+  const int sched = grid <= 0 ? 2 : (strided ? 1 : 0);
   grid = grid <= 0 ? total : std::max(1, std::min(grid, total));
 #define CTR_BPB(ZGV)                                                                       \
   hipLaunchKernelGGL((bp_batch_kernel<true, ZGV, T>), dim3(grid), dim3(kBlock), 0, st, items, \
-                     nscan, total, strided)
+                     nscan, total, sched)
   switch (zg) {
     case 1: CTR_BPB(1); break;
     case 2: CTR_BPB(2); break;
