@@ -50,6 +50,33 @@ __device__ __forceinline__ uint16_t to_bf16_dither(float v, size_t i) {
   return (uint16_t)((__float_as_uint(v) + r) >> 16);
 }
 
+// Global-memory view of a pointer.  Pointers read from memory (the descriptors of the
+// batched kernels) carry no address space, and the compiler then emits flat accesses,
+// which wait on both the scalar and the vector memory counters; kernel arguments are
+// known to be global.  The projector bodies access their operands through gp() so both
+// cases compile to global loads and stores.
+#define CTR_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ CTR_GLOBAL T* gp(T* p) {
+  return (CTR_GLOBAL T*)p;
+}
+// Element i of an array of 4-byte-word structs (ViewP, float2, float4, double2) read
+// through a global pointer.
+template <typename T>
+__device__ __forceinline__ T gload(const T* p, size_t i) {
+  static_assert(sizeof(T) % 4 == 0, "gload: 4-byte words");
+  const CTR_GLOBAL uint32_t* w = (const CTR_GLOBAL uint32_t*)p + i * (sizeof(T) / 4);
+  T v;
+  uint32_t* o = reinterpret_cast<uint32_t*>(&v);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(T) / 4); ++k) o[k] = w[k];
+  return v;
+}
+__device__ __forceinline__ void st(CTR_GLOBAL float* p, size_t i, float v) { p[i] = v; }
+__device__ __forceinline__ void st(CTR_GLOBAL uint16_t* p, size_t i, float v) {
+  p[i] = to_bf16(v);
+}
+
 // Raw buffer view of a float or BF16 array: 32-bit byte offsets, and reads past
 // num_records return 0 instead of touching memory outside the array.
 template <typename T>

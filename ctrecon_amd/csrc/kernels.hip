@@ -45,8 +45,8 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* slot) {
       sa += red[0][i];
       sb += red[1][i];
     }
-    slot[0] = sa;
-    slot[1] = sb;
+    gp(slot)[0] = sa;
+    gp(slot)[1] = sb;
   }
   __syncthreads();  // red[] is reused by the next work item of a persistent block
 }
@@ -76,11 +76,11 @@ __device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk
   if (ray < nray) {
     const int u = COLW ? (ublk * kWaves + wid) : ray / d.DNV;
     const int v = ray - u * d.DNV;
-    const ViewP vp = A.views[p];
-    const float2 bl = A.bnd[(size_t)p * (d.DNU + 1) + u];
-    const float2 br = A.bnd[(size_t)p * (d.DNU + 1) + u + 1];
-    const float4 ct = A.cols[(size_t)p * d.DNU + u];
-    const float Bv = A.Bz[v], Bv1 = A.Bz[v + 1];
+    const ViewP vp = gload(A.views, p);
+    const float2 bl = gload(A.bnd, (size_t)p * (d.DNU + 1) + u);
+    const float2 br = gload(A.bnd, (size_t)p * (d.DNU + 1) + u + 1);
+    const float4 ct = gload(A.cols, (size_t)p * d.DNU + u);
+    const float Bv = gp(A.Bz)[v], Bv1 = gp(A.Bz)[v + 1];
     int NS, NI, sS, cS, mS, mC;
     if (vp.xmarch) {
       NS = d.XN; NI = d.YN; sS = d.ZN; cS = d.XN * d.ZN; mS = 1; mC = d.XN;
@@ -91,7 +91,7 @@ __device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk
     const int ZN1 = d.ZN - 1;
     const float2* me = A.mext + (vp.xmarch ? 0 : d.XN);
     const Buf<TV> V(A.vol, (uint32_t)(d.XN * d.YN * d.ZN));
-    const float* maskf = A.maskf;
+    const CTR_GLOBAL float* maskf = gp(A.maskf);
     float acc = 0.f;
     // one slice: footprint [clo, chi] x [zl, zh] over the cells j0 .. j0 + nj - 1
     auto slice = [&](int ii, float r, float inv, int j0, int nj, float wj0, float wjl) {
@@ -152,7 +152,7 @@ __device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk
           const float ylo = fminf(tl, tr), yhi = fmaxf(tl, tr);
           // clip to the mask's extent on this slice (inside [-0.5, NI - 0.5]); cells
           // outside it are zero
-          const float2 e = me[il];
+          const float2 e = gload(me, il);
           const float clo = fmaxf(ylo, e.x), chi = fminf(yhi, e.y);
           ok = clo < chi;
           if (ok) {
@@ -177,7 +177,7 @@ __device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk
         const float tl = fmaf(fi, bl.y, bl.x);
         const float tr = fmaf(fi, br.y, br.x);
         const float ylo = fminf(tl, tr), yhi = fmaxf(tl, tr);
-        const float2 e = me[ii];
+        const float2 e = gload(me, ii);
         const float clo = fmaxf(ylo, e.x), chi = fminf(yhi, e.y);
         if (!(clo < chi)) continue;
         const float r = fmaf(fi, ct.y, ct.x);
@@ -189,18 +189,18 @@ __device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk
         slice(ii, r, inv, j0, nj, wj0, wjl);
       }
     }
-    const float K = ct.w * __builtin_amdgcn_sqrtf(ct.z + A.h2[v]) * A.invdB[v];
+    const float K = ct.w * __builtin_amdgcn_sqrtf(ct.z + gp(A.h2)[v]) * gp(A.invdB)[v];
     const float val = acc * K;
     const size_t o = ((size_t)k * d.DNU + u) * d.DNV + v;
     if constexpr (FUSED) {
       const size_t og = ((size_t)p * d.DNU + u) * d.DNV + v;
-      const float bb = A.b[og];
+      const float bb = gp(A.b)[og];
       const float diff = bb - val;
-      st(A.out, o, A.invRow ? diff * A.invRow[o] : diff);
+      st(gp(A.out), o, A.invRow ? diff * gp(A.invRow)[o] : diff);
       res = (double)diff * (double)diff;
       bsq = (double)bb * (double)bb;
     } else {
-      st(A.out, o, val);
+      st(gp(A.out), o, val);
     }
   }
   if constexpr (FUSED) block_sum2(res, bsq, slot);
@@ -293,8 +293,8 @@ __device__ __forceinline__ void bp_ref_body(const BpArgs<T>& A, int col, int zbl
   const int zb0 = zblk * (64 * ZG);  // first z of the wave (uniform)
   const int zb = zb0 + lane;
   const int y = col / d.XN, x = col - y * d.XN;
-  float* vcol = A.vol + (size_t)col * d.ZN;
-  if (A.mask[col] != 1) {
+  CTR_GLOBAL float* vcol = gp(A.vol) + (size_t)col * d.ZN;
+  if (gp(A.mask)[col] != 1) {
     if constexpr (!UPDATE) {
 #pragma unroll
       for (int g = 0; g < ZG; ++g)
@@ -329,7 +329,7 @@ __device__ __forceinline__ void bp_ref_body(const BpArgs<T>& A, int col, int zbl
     bool vis = false;
     if (kl < A.vc) {
       const int p = A.vb + kl * A.vs;
-      const ViewP vp = A.views[p];
+      const ViewP vp = gload(A.views, p);
       // row shadow, FP32 (DD3_GPU.cu:1446-1450): t = (dZ +- hz) * S / l + cV; first the
       // whole column's row range, to skip views it does not meet
       const float dXf = Pxf - vp.sx, dYf = Pyf - vp.sy;
@@ -340,7 +340,7 @@ __device__ __forceinline__ void bp_ref_body(const BpArgs<T>& A, int col, int zbl
       const float tb = fmaf(Zhi - vp.sz + c.hz, rlS, c.cV);
       const bool zvis = fmaxf(ta, tb) > -0.5f && fminf(ta, tb) < DNVh;
       // fan-angle shadow, FP64 (DD3_GPU.cu:1447-1464); sy*dX - sx*dY == sy*Px - sx*Py
-      const double2 s64 = A.src64[p];
+      const double2 s64 = gload(A.src64, p);
       const double dX = Px - s64.x, dY = Py - s64.y;
       const double l2 = dX * dX + dY * dY;
       const double rl = rsqrt(l2);
@@ -434,9 +434,10 @@ __device__ __forceinline__ void bp_ref_body(const BpArgs<T>& A, int col, int zbl
     const int z = zb + 64 * g;
     if (z >= d.ZN) continue;
     if constexpr (UPDATE) {
-      const float nx = fmaf(acc[g], A.invCol[(size_t)col * d.ZN + z], vcol[z]);
+      const float nx = fmaf(acc[g], gp(A.invCol)[(size_t)col * d.ZN + z], vcol[z]);
       vcol[z] = nx;
-      if (A.shadow) A.shadow[(size_t)col * d.ZN + z] = to_bf16_dither(nx, (size_t)col * d.ZN + z);
+      if (A.shadow)
+        gp(A.shadow)[(size_t)col * d.ZN + z] = to_bf16_dither(nx, (size_t)col * d.ZN + z);
     } else {
       vcol[z] = acc[g];
     }
