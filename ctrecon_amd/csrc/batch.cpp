@@ -60,10 +60,16 @@ struct ctr_batch {
   bool bf = false;
   std::vector<std::unique_ptr<Scan>> scans;
   Mem sums, zeros;  // sums: [n][osNum][2]; zeros: the largest volume (tuning)
-  Mem fpItems, bpItems, tuneBpItems, redItems;
-  std::vector<int> fpTotal, bpTotal;
+  Mem fpItems, redItems;
+  std::vector<int> fpTotal;
+  // back-projection work lists by z groups per lane (class c = 1..4): the scans of the
+  // class, and per subset step their items (and timing copies with a zero column sum)
+  std::vector<int> clsScans[5];
+  Mem bpItems[5], tuneBpItems[5];
+  std::vector<int> bpTotal[5];
   bool colw = false;
-  int fpGrid = 0, zg = 1, bpGrid = 0;  // grids: persistent workgroups, <= 0 one per item
+  // zg: z groups per lane for every scan, or 0 for each scan's own ceil(ZN / 64)
+  int fpGrid = 0, zg = 0, bpGrid = 0;  // grids: persistent workgroups, <= 0 one per item
   int fpStrided = 0, bpStrided = 0;
   hipStream_t st = nullptr;
   hipGraph_t graph = nullptr;
@@ -79,57 +85,79 @@ struct ctr_batch {
 
 namespace {
 
+int scan_zg(const ctr_batch& B, const Plan& P) {
+  return B.zg > 0 ? B.zg : std::min(4, (P.d.ZN + 63) / 64);
+}
+
 // Descriptor tables for the current tuning (zg and colw change the work lists).
 template <typename T>
 int build_tables(ctr_batch& B) {
   const int n = B.n, S = B.osNum;
   std::vector<FpBatchItem<T>> fp((size_t)S * n);
-  std::vector<BpBatchItem<T>> bp((size_t)S * n), tbp((size_t)S * n);
   std::vector<ReduceItem> red((size_t)S * n);
   B.fpTotal.assign(S, 0);
-  B.bpTotal.assign(S, 0);
-  for (int s = 0; s < S; ++s) {
-    int fb = 0, bb = 0;
-    for (int i = 0; i < n; ++i) {
-      Scan& c = *B.scans[i];
-      const Plan& P = *c.P;
-      const int vc = vcount(P, s, S);
-      const T* vol = B.bf ? (const T*)c.xs.p : (const T*)c.x;
-      T* tp = (T*)c.tp.p;
-      FpBatchItem<T>& f = fp[(size_t)s * n + i];
-      f.a = fp_args<T, T>(P, vol, tp, s, S, c.prj, c.rowInv.f() + c.rowOff[s] * c.frame,
-                       c.partials.d());
-      f.begin = fb;
-      f.upv = fp_blocks_per_view(P, B.colw);
-      fb += f.upv * std::max(vc, 0);
-      red[(size_t)s * n + i] = ReduceItem{c.partials.d(), f.upv * std::max(vc, 0),
-                                          B.sums.d() + ((size_t)i * S + s) * 2};
-      BpBatchItem<T>& b = bp[(size_t)s * n + i];
-      b.a = bp_args<T>(P, tp, c.x, s, S, std::max(vc, 0), c.colInv.f() + c.NV * s,
-                       B.bf ? c.xs.h() : nullptr);
-      b.begin = bb;
-      b.colBlocks = (P.d.XN * P.d.YN + 3) / 4;
-      const int zblocks = (P.d.ZN + 64 * B.zg - 1) / (64 * B.zg);
-      bb += vc > 0 ? b.colBlocks * zblocks : 0;
-      BpBatchItem<T>& t = tbp[(size_t)s * n + i];
-      t = b;
-      t.a.invCol = B.zeros.f();  // x += acc * 0: timing runs leave the iterate unchanged
-    }
-    B.fpTotal[s] = fb;
-    B.bpTotal[s] = bb;
+  for (int c = 1; c <= 4; ++c) {
+    B.clsScans[c].clear();
+    B.bpTotal[c].assign(S, 0);
   }
+  for (int i = 0; i < n; ++i) B.clsScans[scan_zg(B, *B.scans[i]->P)].push_back(i);
   auto up = [&](Mem& m, const void* src, size_t bytes) -> int {
-    if (!m.p) {
-      int rc = m.alloc(bytes);
-      if (rc) return rc;
+    if (m.p) {
+      (void)hipFree(m.p);
+      m.p = nullptr;
     }
+    int rc = m.alloc(bytes);
+    if (rc) return rc;
     CTR_HIP(hipMemcpy(m.p, src, bytes, hipMemcpyHostToDevice));
     return CTR_OK;
   };
+  for (int s = 0; s < S; ++s) {
+    int fb = 0;
+    for (int i = 0; i < n; ++i) {
+      Scan& c = *B.scans[i];
+      const Plan& P = *c.P;
+      const int vc = std::max(vcount(P, s, S), 0);
+      const T* vol = B.bf ? (const T*)c.xs.p : (const T*)c.x;
+      FpBatchItem<T>& f = fp[(size_t)s * n + i];
+      f.a = fp_args<T, T>(P, vol, (T*)c.tp.p, s, S, c.prj,
+                          c.rowInv.f() + c.rowOff[s] * c.frame, c.partials.d());
+      f.begin = fb;
+      f.upv = fp_blocks_per_view(P, B.colw);
+      fb += f.upv * vc;
+      red[(size_t)s * n + i] = ReduceItem{c.partials.d(), f.upv * vc,
+                                          B.sums.d() + ((size_t)i * S + s) * 2};
+    }
+    B.fpTotal[s] = fb;
+  }
   int rc;
+  for (int cz = 1; cz <= 4; ++cz) {
+    const std::vector<int>& L = B.clsScans[cz];
+    if (L.empty()) continue;
+    const size_t m = L.size();
+    std::vector<BpBatchItem<T>> bp((size_t)S * m), tbp((size_t)S * m);
+    for (int s = 0; s < S; ++s) {
+      int bb = 0;
+      for (size_t j = 0; j < m; ++j) {
+        Scan& c = *B.scans[L[j]];
+        const Plan& P = *c.P;
+        const int vc = std::max(vcount(P, s, S), 0);
+        BpBatchItem<T>& b = bp[(size_t)s * m + j];
+        b.a = bp_args<T>(P, (const T*)c.tp.p, c.x, s, S, vc, c.colInv.f() + c.NV * s,
+                         B.bf ? c.xs.h() : nullptr);
+        b.begin = bb;
+        b.colBlocks = (P.d.XN * P.d.YN + 3) / 4;
+        const int zblocks = (P.d.ZN + 64 * cz - 1) / (64 * cz);
+        bb += vc > 0 ? b.colBlocks * zblocks : 0;
+        BpBatchItem<T>& t = tbp[(size_t)s * m + j];
+        t = b;
+        t.a.invCol = B.zeros.f();  // x += acc * 0: timing runs leave the iterate unchanged
+      }
+      B.bpTotal[cz][s] = bb;
+    }
+    if ((rc = up(B.bpItems[cz], bp.data(), bp.size() * sizeof(bp[0])))) return rc;
+    if ((rc = up(B.tuneBpItems[cz], tbp.data(), tbp.size() * sizeof(tbp[0])))) return rc;
+  }
   if ((rc = up(B.fpItems, fp.data(), fp.size() * sizeof(fp[0])))) return rc;
-  if ((rc = up(B.bpItems, bp.data(), bp.size() * sizeof(bp[0])))) return rc;
-  if ((rc = up(B.tuneBpItems, tbp.data(), tbp.size() * sizeof(tbp[0])))) return rc;
   return up(B.redItems, red.data(), red.size() * sizeof(red[0]));
 }
 
@@ -143,9 +171,15 @@ int step_fp(ctr_batch& B, int s) {
 
 template <typename T>
 int step_bp(ctr_batch& B, int s, bool tuning) {
-  const Mem& m = tuning ? B.tuneBpItems : B.bpItems;
-  return launch_bp_batch<T>((const BpBatchItem<T>*)m.p + (size_t)s * B.n, B.n, B.bpTotal[s],
-                            B.zg, B.bpGrid, B.bpStrided, B.st);
+  for (int cz = 1; cz <= 4; ++cz) {
+    const size_t m = B.clsScans[cz].size();
+    if (!m) continue;
+    const Mem& mem = tuning ? B.tuneBpItems[cz] : B.bpItems[cz];
+    int rc = launch_bp_batch<T>((const BpBatchItem<T>*)mem.p + (size_t)s * m, (int)m,
+                                B.bpTotal[cz][s], cz, B.bpGrid, B.bpStrided, B.st);
+    if (rc) return rc;
+  }
+  return CTR_OK;
 }
 
 int sweep_launches(ctr_batch& B) {
@@ -220,10 +254,10 @@ int autotune(ctr_batch& B) {
   B.fpGrid = bgrid;
   B.fpStrided = bstr;
   best = 1e30f;
-  int bzg = 1;
+  int bzg = 0;
   bgrid = 0;
   bstr = 0;
-  const int zgs[3] = {1, 2, 4};
+  const int zgs[4] = {0, 1, 2, 4};  // 0: each scan's own ceil(ZN / 64)
   for (int zg : zgs) {
     if (zg > 1 && 64 * (zg / 2) >= maxZN) break;  // more groups than any column needs
     B.zg = zg;
@@ -361,7 +395,7 @@ int ctr_batch_set_graph(ctr_batch* B, int enable) {
 int ctr_batch_set_tuning(ctr_batch* B, int fp_colw, int fp_grid, int fp_strided, int bp_zg,
                          int bp_grid, int bp_strided) {
   if (!B) return fail(CTR_ERR_ARG, "NULL batch");
-  if (bp_zg < 1 || bp_zg > 4 || fp_grid < 0 || bp_grid < 0)
+  if (bp_zg < 0 || bp_zg > 4 || fp_grid < 0 || bp_grid < 0)
     return fail(CTR_ERR_ARG, "bad tuning");
   if (fp_colw)
     for (auto& c : B->scans)
@@ -391,7 +425,8 @@ int ctr_batch_get_info(const ctr_batch* B, ctr_batch_info* info) {
   long long f = 0, b = 0;
   for (int s = 0; s < B->osNum; ++s) {
     f += B->fpTotal[s];
-    b += B->bpTotal[s];
+    for (int cz = 1; cz <= 4; ++cz)
+      if (!B->bpTotal[cz].empty()) b += B->bpTotal[cz][s];
   }
   info->fp_items = f;
   info->bp_items = b;

@@ -125,7 +125,8 @@ def test_tuned_batch_matches_per_scan(gpu_plan_factory):
         assert e <= 1e-5
         assert np.allclose(res[:, i], r_ref, rtol=1e-5)
     # every tuning candidate computes the same iterates
-    for colw, zg, grid, strided in [(True, 1, 1024, False), (False, 2, 2048, True),
+    for colw, zg, grid, strided in [(True, 0, 0, False), (True, 1, 1024, False),
+                                    (False, 2, 2048, True),
                                     (True, 4, 0, False), (False, 1, 0, True),
                                     (True, 2, 512, True)]:
         for x in xs:
