@@ -123,3 +123,46 @@ def test_matlab_surface_validates_shapes():
     assert D.projector_mode("fp_gpu_branchless") == 0
     assert D.projector_mode("bp_gpu_matched") == 16
     assert D.projector_mode("bp_gpu_zline") == 3
+
+
+def test_null_handles_of_the_newer_entry_points():
+    """Sessions, batches, the network, storage and mask entry points reject NULL
+    handles and buffers with CTR_ERR_ARG (before any device work)."""
+    L = lib()
+    p = ctypes.c_void_p()
+    assert L.ctr_plan_set_storage(None, 0) == _lib.CTR_ERR_ARG
+    assert L.ctr_plan_mask_volume(None, None, None) == _lib.CTR_ERR_ARG
+    assert L.ctr_plan_fp_bf16(None, None, None, 0, 1, 1, None) == _lib.CTR_ERR_ARG
+    assert L.ctr_plan_bp_bf16(None, None, None, 0, 1, 1, 0, None) == _lib.CTR_ERR_ARG
+    assert L.ctr_f32_to_bf16(None, None, 4, None) == _lib.CTR_ERR_ARG
+    assert L.ctr_bf16_to_f32(None, None, 4, None) == _lib.CTR_ERR_ARG
+    assert L.ctr_device_sub(None, None, None, 4, None) == _lib.CTR_ERR_ARG
+    assert L.ctr_device_axpy(None, None, 1.0, 4, None) == _lib.CTR_ERR_ARG
+    assert L.ctr_sart_create(None, None, 1, 0, ctypes.byref(p)) == _lib.CTR_ERR_ARG
+    assert L.ctr_sart_create(None, None, 1, 0, None) == _lib.CTR_ERR_ARG
+    assert L.ctr_sart_sweeps(None, None, 1, 0.0, None, None, None) == _lib.CTR_ERR_ARG
+    assert L.ctr_sart_destroy(None) == _lib.CTR_OK
+    assert L.ctr_batch_create(None, 0, None, None, 1, 0, ctypes.byref(p)) == _lib.CTR_ERR_ARG
+    assert L.ctr_batch_sweeps(None, 1, 0.0, None, None, None) == _lib.CTR_ERR_ARG
+    assert L.ctr_batch_set_graph(None, 1) == _lib.CTR_ERR_ARG
+    assert L.ctr_batch_set_tuning(None, 1, 0, 0, 0, 0, 0) == _lib.CTR_ERR_ARG
+    assert L.ctr_batch_get_info(None, None) == _lib.CTR_ERR_ARG
+    assert L.ctr_batch_destroy(None) == _lib.CTR_OK
+    assert L.ctr_unet_create(0, None) == _lib.CTR_ERR_ARG
+    assert L.ctr_unet_set_params(None, None, 0) == _lib.CTR_ERR_ARG
+    assert L.ctr_unet_get_params(None, None, 0) == _lib.CTR_ERR_ARG
+    assert L.ctr_unet_set_scales(None, 1.0, 1.0, 1.0) == _lib.CTR_ERR_ARG
+    assert L.ctr_unet_forward(None, None, None, None, 4, 4, 32, None) == _lib.CTR_ERR_ARG
+    assert L.ctr_unet_loss_grad(None, None, None, None, 4, 4, 32, None, None) == _lib.CTR_ERR_ARG
+    assert L.ctr_unet_train_step(None, None, None, None, 4, 4, 32, 1e-3, None) == _lib.CTR_ERR_ARG
+    assert L.ctr_unet_destroy(None) == _lib.CTR_OK
+
+
+def test_unet_parameter_count_matches_the_layer_table():
+    from ctrecon_amd.unet import LAYERS, num_params
+
+    assert lib().ctr_unet_num_params() == num_params()
+    assert num_params() == sum(27 * ci * co + co for ci, co in LAYERS) + 17
+    from oracle import unet_oracle as U
+
+    assert U.LAYERS == LAYERS
