@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libctrecon_amd.so")
 ARCH = os.environ.get("CTRECON_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "sat.hip", "plan.cpp", "batch.cpp", "unet.hip", "unet.cpp"]
+SOURCES = ["kernels.hip", "sat.hip", "pdd.hip", "plan.cpp", "batch.cpp", "unet.hip", "unet.cpp"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",

@@ -131,13 +131,16 @@ struct Plan {
   float* Bz = nullptr;         // [DNV+1]      (bz_v - z0)/dz
   float* h2 = nullptr;         // [DNV]        (zds_v - z0)^2
   float* invdB = nullptr;      // [DNV]        1/(Bz[v+1]-Bz[v])
+  double2* rot64 = nullptr;    // [PN]         (cos, sin) of the regularised view angle, FP64
+  double2* det64 = nullptr;    // [DNU]        detector cell centres (xds, yds), FP64
+  float* zc = nullptr;         // [DNV]        (zds_v - z0)/dz
   uint8_t* mask = nullptr;     // [XN*YN]
   float* maskf = nullptr;      // [XN*YN]      mask byte as float (the forward multiplier)
   float2* mext = nullptr;      // [XN + YN]    clip interval of the mask per x- then y-slice
   // host copies (for culling and solvers)
   std::vector<ViewP> h_views;
   HostGeo hg;
-  // projector method: CTR_METHOD_DIRECT or CTR_METHOD_SAT
+  // projector method: CTR_METHOD_DIRECT, CTR_METHOD_SAT or CTR_METHOD_PSEUDO_DD
   int method = CTR_METHOD_DIRECT;
   // storage of the solvers' projector operands: CTR_STORAGE_F32, CTR_STORAGE_BF16 or
   // CTR_STORAGE_BF16_SINO
@@ -207,6 +210,11 @@ int launch_dot(const float* a, const float* b, size_t n, double* partials, int n
                double* result, hipStream_t st);
 int launch_reduce_pairs(const double* partials, int nparts, double* result, hipStream_t st);
 int fp_partial_blocks(const Plan& P, int vc);
+// pseudo distance-driven pair (pdd.hip), FP32 operands
+int launch_fp_pdd(const Plan& P, const float* vol, float* out, int vb, int vs, int vc,
+                  hipStream_t st);
+int launch_bp_pdd(const Plan& P, const float* prj, float* vol, int vb, int vs, int vc,
+                  hipStream_t st);
 // integral-image method (sat.hip)
 size_t sat_volume_floats(const Dims& d);
 size_t sat_sino_floats(const Dims& d, int vc);

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "ctr_internal.hpp"
@@ -87,7 +88,10 @@ int validate_geometry(const ctr_geometry* g) {
 }  // namespace
 
 // Builds every table of the plan in FP64 on the host.
-static int build_tables(const ctr_geometry* g, Plan& P) {
+// half: 0, or 0.5 to move the footprints of the forward projector half a cell along the
+// in-plane axis across the rays and along z (the sampling of the reference's FP64
+// branchless projector, DD3_GPU.cu:350-360 / 414-422).
+static int build_tables(const ctr_geometry* g, Plan& P, double half = 0.0) {
   const int DNU = g->DNU, DNV = g->DNV, PN = g->PN;
   const double x0 = g->x0, y0 = g->y0, z0 = g->z0, dx = g->dx, dz = g->dz;
   Dims& d = P.d;
@@ -158,7 +162,10 @@ static int build_tables(const ctr_geometry* g, Plan& P) {
 
   // per-view tables
   std::vector<ViewP> views(PN);
-  std::vector<double2> src64(PN);
+  std::vector<double2> src64(PN), rot64(PN), det64(DNU);
+  std::vector<float> zc(DNV);
+  for (int u = 0; u < DNU; ++u) det64[u] = make_double2(g->xds[u], g->yds[u]);
+  for (int v = 0; v < DNV; ++v) zc[v] = (float)((g->zds[v] - z0) / dz);
   std::vector<float2> bnd((size_t)PN * (DNU + 1));
   std::vector<float4> cols((size_t)PN * DNU);
   for (int p = 0; p < PN; ++p) {
@@ -168,7 +175,8 @@ static int build_tables(const ctr_geometry* g, Plan& P) {
     ViewP& vp = views[p];
     vp.sx = (float)sx; vp.sy = (float)sy; vp.sz = (float)sz;
     src64[p] = make_double2(sx, sy);
-    vp.az = (float)(sz / dz + ocz);
+    rot64[p] = make_double2(c, s);
+    vp.az = (float)(sz / dz + ocz + half);
     vp.xmarch = std::fabs(c) <= std::fabs(s) ? 1 : 0;  // DD3_GPU.cu:914
     vp.wx = std::fabs(sx) > std::fabs(sy) ? 1 : 0;  // DD3_GPU.cu:1453
     vp.pad1 = 0;
@@ -179,10 +187,10 @@ static int build_tables(const ctr_geometry* g, Plan& P) {
       double f, a;
       if (vp.xmarch) {
         f = (By - sy) / (Bx - sx);
-        a = ((-ocx * dx - Bx) * f + By) / dx + ocy;
+        a = ((-ocx * dx - Bx) * f + By) / dx + ocy + half;
       } else {
         f = (Bx - sx) / (By - sy);
-        a = ((-ocy * dx - By) * f + Bx) / dx + ocx;
+        a = ((-ocy * dx - By) * f + Bx) / dx + ocx + half;
       }
       bnd[(size_t)p * (DNU + 1) + b] = make_float2((float)a, (float)f);
     }
@@ -247,6 +255,9 @@ static int build_tables(const ctr_geometry* g, Plan& P) {
   if ((rc = upload(&P.maskf, maskf))) return rc;
   if ((rc = upload(&P.views, views))) return rc;
   if ((rc = upload(&P.src64, src64))) return rc;
+  if ((rc = upload(&P.rot64, rot64))) return rc;
+  if ((rc = upload(&P.det64, det64))) return rc;
+  if ((rc = upload(&P.zc, zc))) return rc;
   if ((rc = upload(&P.bnd, bnd))) return rc;
   if ((rc = upload(&P.cols, cols))) return rc;
   if ((rc = upload(&P.Bz, Bz))) return rc;
@@ -260,6 +271,9 @@ static void free_plan(Plan* P) {
   if (!P) return;
   (void)hipFree(P->views);
   (void)hipFree(P->src64);
+  (void)hipFree(P->rot64);
+  (void)hipFree(P->det64);
+  (void)hipFree(P->zc);
   (void)hipFree(P->bnd);
   (void)hipFree(P->cols);
   (void)hipFree(P->Bz);
@@ -321,6 +335,14 @@ static int ensure_buf(float** p, size_t* have, size_t need, const char* what) {
 template <typename TV, typename TO = TV>
 static int fp_any(Plan& P, const TV* vol, TO* out, int vb, int vs, int vc, hipStream_t st,
                   const float* b, const float* invRow, double* partials) {
+  if (P.method == CTR_METHOD_PSEUDO_DD) {
+    if constexpr (std::is_same<TV, float>::value && std::is_same<TO, float>::value) {
+      if (b) return fail(CTR_ERR_UNSUPPORTED, "the pseudo-DD projector has no fused residual");
+      return launch_fp_pdd(P, vol, out, vb, vs, vc, st);
+    } else {
+      return fail(CTR_ERR_UNSUPPORTED, "the pseudo-DD projector takes FP32 operands");
+    }
+  }
   if (P.method == CTR_METHOD_SAT) {
     int rc = ensure_buf(&P.sat_vol, &P.sat_vol_n, 2 * sat_volume_floats(P.d),
                         "volume integral images");
@@ -336,6 +358,17 @@ template <typename T>
 static int bp_any(Plan& P, const T* prj, float* vol, int vb, int vs, int vc, int kind,
                   hipStream_t st, const float* invCol, int accumulate,
                   uint16_t* shadow = nullptr) {
+  if (P.method == CTR_METHOD_PSEUDO_DD) {
+    if (kind != CTR_BP_REFERENCE)
+      return fail(CTR_ERR_UNSUPPORTED, "the pseudo-DD pair has no matched adjoint");
+    if (accumulate)
+      return fail(CTR_ERR_UNSUPPORTED, "the pseudo-DD back projector has no update form");
+    if constexpr (std::is_same<T, float>::value) {
+      return launch_bp_pdd(P, prj, vol, vb, vs, vc, st);
+    } else {
+      return fail(CTR_ERR_UNSUPPORTED, "the pseudo-DD projector takes FP32 operands");
+    }
+  }
   if (kind == CTR_BP_REFERENCE && P.method == CTR_METHOD_SAT) {
     int rc = ensure_buf(&P.sat_sino, &P.sat_sino_n, 2 * sat_sino_floats(P.d, vc),
                         "sinogram integral images");
@@ -350,6 +383,15 @@ static int bp_any(Plan& P, const T* prj, float* vol, int vb, int vs, int vc, int
                             shadow);
   }
   return launch_bp<T>(P, prj, vol, vb, vs, vc, kind, st, invCol, accumulate, shadow);
+}
+
+// The OS-SART and CG drivers run the reference's mode-0 model (os_sart.m:16-27 and
+// conjgrad.m call the branchless projectors only).
+static int check_solver_method(const Plan& P) {
+  if (P.method == CTR_METHOD_PSEUDO_DD)
+    return fail(CTR_ERR_UNSUPPORTED,
+                "the OS-SART / CG drivers use the branchless model (method DIRECT or SAT)");
+  return CTR_OK;
 }
 
 // Owns a set of HIP events.
@@ -428,13 +470,18 @@ ctr_geometry make_geometry(float x0, float y0, float z0, int DNU, int DNV, const
   return g;
 }
 
-// prjMode handling of DD3Proj_gpu / DD3Back_gpu (DD3_GPU.cu:1144-1183, 1778-1815):
-// 0 and every value outside 1..3 select the branchless model.
-int check_mode(int mode) {
-  if (mode == 1 || mode == 2 || mode == 3)
-    return fail(CTR_ERR_UNSUPPORTED, "prjMode " + std::to_string(mode) +
-                                         " (pseudo-DD / double / z-line) is not implemented");
-  return CTR_OK;
+// prjMode dispatch of DD3Proj_gpu / DD3Back_gpu (DD3_GPU.cu:1144-1183, 1778-1815):
+// what each value selects, every value outside 1..3 being mode 0.
+enum class Model { Branchless, PseudoDD, Fp64Sat, ZLine };
+Model fp_model(int mode) {
+  if (mode == 1 || mode == 3) return Model::PseudoDD;
+  if (mode == 2) return Model::Fp64Sat;
+  return Model::Branchless;
+}
+Model bp_model(int mode) {
+  if (mode == 1 || mode == 2) return Model::PseudoDD;
+  if (mode == 3) return Model::ZLine;
+  return Model::Branchless;
 }
 
 struct PlanGuard {
@@ -482,8 +529,9 @@ int ctr_plan_destroy(ctr_plan* plan) {
 int ctr_plan_set_method(ctr_plan* plan, int method) {
   Plan* P = get(plan);
   if (!P) return fail(CTR_ERR_ARG, "NULL plan");
-  if (method != CTR_METHOD_DIRECT && method != CTR_METHOD_SAT)
-    return fail(CTR_ERR_ARG, "method must be CTR_METHOD_DIRECT or CTR_METHOD_SAT");
+  if (method != CTR_METHOD_DIRECT && method != CTR_METHOD_SAT && method != CTR_METHOD_PSEUDO_DD)
+    return fail(CTR_ERR_ARG,
+                "method must be CTR_METHOD_DIRECT, CTR_METHOD_SAT or CTR_METHOD_PSEUDO_DD");
   P->method = method;
   return CTR_OK;
 }
@@ -569,6 +617,7 @@ int ctr_bf16_to_f32(float* d_dst, const unsigned short* d_src, unsigned long lon
 int ctr_residual(ctr_plan* plan, const float* d_x, const float* d_prj, double* out) {
   Plan* P = get(plan);
   if (!P || !d_x || !d_prj || !out) return fail(CTR_ERR_ARG, "NULL argument");
+  if (int rc0 = check_solver_method(*P)) return rc0;
   CTR_HIP(hipSetDevice(P->device));
   const int PN = P->d.PN;
   // chunk the views so the scratch stays modest
@@ -620,6 +669,7 @@ int ctr_sart_create(ctr_plan* plan, const float* d_prj, int osNum, int bp_kind, 
   if (osNum < 1 || osNum > P->d.PN) return fail(CTR_ERR_ARG, "osNum must be in [1, PN]");
   if (bp_kind != CTR_BP_REFERENCE && bp_kind != CTR_BP_MATCHED)
     return fail(CTR_ERR_ARG, "bad bp_kind");
+  if (int rc0 = check_solver_method(*P)) return rc0;
   CTR_HIP(hipSetDevice(P->device));
   std::unique_ptr<ctr_sart> S(new ctr_sart());
   S->P = P;
@@ -802,6 +852,7 @@ int ctr_conjgrad(ctr_plan* plan, float* d_x, const float* d_prj, int iterNum, in
   if (iterNum < 0) return fail(CTR_ERR_ARG, "iterNum must be >= 0");
   if (bp_kind != CTR_BP_REFERENCE && bp_kind != CTR_BP_MATCHED)
     return fail(CTR_ERR_ARG, "bad bp_kind");
+  if (int rc0 = check_solver_method(*P)) return rc0;
   CTR_HIP(hipSetDevice(P->device));
   const Dims& d = P->d;
   const size_t NV = (size_t)d.XN * d.YN * d.ZN, NS = (size_t)d.DNU * d.DNV * d.PN;
@@ -878,13 +929,26 @@ int ctr_dd3_proj(float x0, float y0, float z0, int DNU, int DNV, const float* xd
                  float imgZCenter, const float* hangs, const float* hzPos, int PN, int XN,
                  int YN, int ZN, const float* hvol, float* hprj, float dx, float dz,
                  const unsigned char* mask, int gpunum, int prjMode) {
-  int rc = check_mode(prjMode);
-  if (rc) return rc;
   if (!hvol || !hprj) return fail(CTR_ERR_ARG, "NULL volume or projection buffer");
+  const Model m = fp_model(prjMode);
   ctr_geometry g = make_geometry(x0, y0, z0, DNU, DNV, xds, yds, zds, imgXCenter, imgYCenter,
                                  imgZCenter, hangs, hzPos, PN, XN, YN, ZN, dx, dz, mask);
   PlanGuard pg;
-  if ((rc = ctr_plan_create(&g, gpunum, &pg.p))) return rc;
+  int rc;
+  if (m == Model::Fp64Sat) {
+    // the branchless footprint model, sampled half a cell off (build_tables)
+    if ((rc = validate_geometry(&g)) || (rc = set_device(gpunum))) return rc;
+    Plan* P = new Plan();
+    P->device = gpunum;
+    if ((rc = build_tables(&g, *P, 0.5))) {
+      free_plan(P);
+      return rc;
+    }
+    pg.p = new ctr_plan{P};
+  } else if ((rc = ctr_plan_create(&g, gpunum, &pg.p))) {
+    return rc;
+  }
+  if (m == Model::PseudoDD && (rc = ctr_plan_set_method(pg.p, CTR_METHOD_PSEUDO_DD))) return rc;
   const size_t NV = (size_t)XN * YN * ZN, NS = (size_t)DNU * DNV * PN;
   DBuf dv, dp;
   if ((rc = dv.alloc(NV * sizeof(float)))) return rc;
@@ -901,20 +965,27 @@ int ctr_dd3_back(float x0, float y0, float z0, int DNU, int DNV, const float* xd
                  int YN, int ZN, float* hvol, const float* hprj, float dx, float dz,
                  const unsigned char* mask, int gpunum, int squared, int prjMode) {
   (void)squared;  // ignored, as in the reference (DD3_GPU.cu:1393)
-  const int kind = (prjMode & 16) ? CTR_BP_MATCHED : CTR_BP_REFERENCE;
-  int rc = check_mode(prjMode & ~16);
-  if (rc) return rc;
   if (!hvol || !hprj) return fail(CTR_ERR_ARG, "NULL volume or projection buffer");
+  const bool matched = (prjMode & 16) != 0;
+  const Model m = bp_model(prjMode & ~16);
+  if (matched && m != Model::Branchless)
+    return fail(CTR_ERR_UNSUPPORTED, "the matched adjoint exists for the branchless model only");
+  if (m == Model::ZLine)
+    return fail(CTR_ERR_UNSUPPORTED, "prjMode 3 (z-line back projector) is not implemented");
   ctr_geometry g = make_geometry(x0, y0, z0, DNU, DNV, xds, yds, zds, imgXCenter, imgYCenter,
                                  imgZCenter, hangs, hzPos, PN, XN, YN, ZN, dx, dz, mask);
   PlanGuard pg;
+  int rc;
   if ((rc = ctr_plan_create(&g, gpunum, &pg.p))) return rc;
+  if (m == Model::PseudoDD && (rc = ctr_plan_set_method(pg.p, CTR_METHOD_PSEUDO_DD))) return rc;
   const size_t NV = (size_t)XN * YN * ZN, NS = (size_t)DNU * DNV * PN;
   DBuf dv, dp;
   if ((rc = dv.alloc(NV * sizeof(float)))) return rc;
   if ((rc = dp.alloc(NS * sizeof(float)))) return rc;
   CTR_HIP(hipMemcpy(dp.p, hprj, NS * sizeof(float), hipMemcpyHostToDevice));
-  if ((rc = ctr_plan_bp(pg.p, dp.f(), dv.f(), 0, 1, PN, kind, nullptr))) return rc;
+  if ((rc = ctr_plan_bp(pg.p, dp.f(), dv.f(), 0, 1, PN,
+                        matched ? CTR_BP_MATCHED : CTR_BP_REFERENCE, nullptr)))
+    return rc;
   CTR_HIP(hipMemcpy(hvol, dv.p, NV * sizeof(float), hipMemcpyDeviceToHost));
   return CTR_OK;
 }

@@ -9,9 +9,13 @@ shapes and column-major (Fortran) memory order:
 * sinogram ``[row_count, total_col_count, nviews]`` (detector row fastest)
 
 ``view_ind`` is zero-based (MATLAB's is one-based).  The projector strings of
-dd3.m:233-246 / :291-304 are accepted; the GPU branchless model
-(``*_branchless*``, prjMode 0) is what runs.  ``bp_gpu_matched`` (an addition of
-this library) selects the exact adjoint of the forward projector.
+dd3.m:233-246 / :291-304 select prjMode as there, and prjMode selects the model as
+DD3Proj_gpu / DD3Back_gpu do (DD3_GPU.cu:1144-1183, 1778-1815): pseudo distance-driven
+for ``fp_gpu_volume_rendering`` / ``fp_gpu_pseudo_dd`` and ``bp_gpu_pseudo_dd`` /
+``bp_gpu_soft_interp``, the FP64 branchless forward model for ``fp_gpu_soft_interp``,
+the z-line back projector for ``bp_gpu_zline``, the branchless model otherwise.
+``bp_gpu_matched`` (an addition of this library) selects the exact adjoint of the
+branchless forward projector.
 """
 
 from __future__ import annotations

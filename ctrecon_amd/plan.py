@@ -156,8 +156,11 @@ class Plan:
         self.handle = h.value
 
     def set_method(self, method):
-        """'direct' (footprint integration, default) or 'sat' (compensated integral images)."""
-        code = {"direct": _lib.CTR_METHOD_DIRECT, "sat": _lib.CTR_METHOD_SAT}[method]
+        """'direct' (footprint integration, default), 'sat' (compensated integral images)
+        -- both the branchless distance-driven model -- or 'pseudo_dd' (the reference's
+        pseudo distance-driven pair, prjMode 1)."""
+        code = {"direct": _lib.CTR_METHOD_DIRECT, "sat": _lib.CTR_METHOD_SAT,
+                "pseudo_dd": _lib.CTR_METHOD_PSEUDO_DD}[method]
         check(lib().ctr_plan_set_method(self.handle, code))
         self.method = method
 

@@ -30,6 +30,9 @@ extern "C" {
 
 /* ---- projector modes (prjMode argument of the reference, iterativeRecon/dd3.m:233-246) ---- */
 #define CTR_MODE_BRANCHLESS 0   /* distance-driven, branchless (the reference's mode 0 / default) */
+#define CTR_MODE_PSEUDO_DD 1    /* forward and back: pseudo distance-driven (DD3_GPU.cu:1153, 1787) */
+#define CTR_MODE_FP64_SAT 2     /* forward: FP64 branchless (DD3_GPU.cu:1161); back: pseudo-DD (:1794) */
+#define CTR_MODE_ZLINE 3        /* forward: pseudo-DD (:1168); back: z-line branchless (:1801) */
 /* Back projector selector (build addition, no reference mode number): the exact
  * transpose of the mode-0 forward projector, used for adjoint tests and CG. */
 #define CTR_BP_REFERENCE 0
@@ -42,6 +45,12 @@ extern "C" {
  * the same operator; the matched adjoint is always DIRECT. */
 #define CTR_METHOD_DIRECT 0
 #define CTR_METHOD_SAT 1
+/* The reference's pseudo distance-driven pair (prjMode 1 / 3 forward, 1 / 2 back;
+ * iterativeRecon/DD3_GPU.cu:673-834, 1478-1528): one ray per detector cell sampled
+ * bilinearly on the slice centre planes, and a voxel-driven bilinear sinogram sample per
+ * view.  A different operator from the two above; FP32 operands only, reference back
+ * projector only (no matched adjoint), not available to the OS-SART / CG drivers. */
+#define CTR_METHOD_PSEUDO_DD 2
 
 /* Storage of projector operands (build addition; the reference is FP32 throughout).
  * BF16 arrays are unsigned short bit patterns: the upper half of the FP32 value,
@@ -52,11 +61,13 @@ extern "C" {
 
 /*
  * Forward projection, host buffers in and out.
- * Replaces DD3Proj_gpu (iterativeRecon/DD3_GPU.cu:1132-1142), same parameter list.
- * Unlike the reference, hvol is not modified (the mask is applied on the device)
- * and the device is not reset.  prjMode 0 (and any value the reference maps to
- * its default branch) selects the branchless distance-driven model; modes 1-3
- * return CTR_ERR_UNSUPPORTED.
+ * Replaces DD3Proj_gpu (iterativeRecon/DD3_GPU.cu:1132-1142), same parameter list and
+ * the same prjMode dispatch (:1144-1183): 1 and 3 the pseudo distance-driven projector,
+ * 2 the FP64 branchless projector (computed here in FP32: the footprint model of mode 0
+ * sampled half a cell off across the rays and along z, as the reference's point-sampled
+ * integral images are), 0 and every other value the branchless distance-driven model.
+ * Unlike the reference, hvol is not modified (the mask is applied on the device) and
+ * the device is not reset.
  */
 int ctr_dd3_proj(float x0, float y0, float z0, int DNU, int DNV, const float* xds,
                  const float* yds, const float* zds, float imgXCenter, float imgYCenter,
@@ -67,9 +78,11 @@ int ctr_dd3_proj(float x0, float y0, float z0, int DNU, int DNV, const float* xd
 /*
  * Back projection, host buffers in and out.
  * Replaces DD3Back_gpu (iterativeRecon/DD3_GPU.cu:1766-1775), same parameter list
- * (squared is accepted and ignored, as in the reference, :1393).  prjMode 0 is the
- * reference-arithmetic branchless back projector; bit 4 (prjMode | 16) selects the
- * matched adjoint instead.  Voxels whose mask byte is not 1 are 0 (:1417).
+ * (squared is accepted and ignored, as in the reference, :1393) and the same prjMode
+ * dispatch (:1778-1815): 1 and 2 the pseudo distance-driven back projector, 3 the z-line
+ * branchless back projector, 0 and every other value the reference-arithmetic
+ * branchless back projector; bit 4 (0 | 16) selects the matched adjoint of the mode-0
+ * forward projector instead.  Voxels whose mask byte is not 1 are 0 (:1417, :1496).
  */
 int ctr_dd3_back(float x0, float y0, float z0, int DNU, int DNV, const float* xds,
                  const float* yds, const float* zds, float imgXCenter, float imgYCenter,
@@ -104,9 +117,10 @@ typedef struct ctr_plan ctr_plan;
 int ctr_plan_create(const ctr_geometry* geom, int device, ctr_plan** out);
 int ctr_plan_destroy(ctr_plan* plan);
 
-/* Select the projector method (CTR_METHOD_DIRECT, the default, or CTR_METHOD_SAT)
- * used by every later call on the plan.  No reference counterpart: the reference
- * always builds integral images (iterativeRecon/DD3_GPU.cu:1036, 1710). */
+/* Select the projector method (CTR_METHOD_DIRECT, the default, CTR_METHOD_SAT or
+ * CTR_METHOD_PSEUDO_DD) used by every later call on the plan.  The reference selects
+ * the model per call with prjMode and always builds integral images for mode 0
+ * (iterativeRecon/DD3_GPU.cu:1036, 1710). */
 int ctr_plan_set_method(ctr_plan* plan, int method);
 
 /* Storage the OS-SART and CG drivers use for their projector operands (build

@@ -17,6 +17,9 @@
 //   detector parameters    iterativeRecon/DD3_GPU.cu:1189-1217 (calDetParas)
 //   sinogram SAT           iterativeRecon/DD3_GPU.cu:1241-1323
 //   back (mode 0)          iterativeRecon/DD3_GPU.cu:1397-1475, host 1651-1760
+//   forward, pseudo-DD     iterativeRecon/DD3_GPU.cu:673-834 (prjMode 1 and 3)
+//   forward, FP64 SAT      iterativeRecon/DD3_GPU.cu:255-660 (prjMode 2)
+//   back, pseudo-DD        iterativeRecon/DD3_GPU.cu:1478-1528 (prjMode 1 and 2)
 //   OS-SART                iterativeRecon/os_sart.m:13-44
 //   CG                     iterativeRecon/conjgrad.m:4-27
 //
@@ -504,6 +507,154 @@ void bp_direct(const Geo<T>& g, const TI* prj, TO* vol, int nthreads) {
     }
 }
 
+// ---------------- pseudo distance driven (prjMode 1 / 3 forward, 1 / 2 back) ----------------
+// Linear-filtered fetch with border addressing (texels outside the array read 0) at
+// texel-index coordinates (a, b) along W and H, i.e. the unnormalised texture coordinate
+// minus 0.5.  f(w, h) returns texel (w, h).
+template <typename T, typename F>
+inline T lerp2_border(const F& f, int W, int H, T a, T b) {
+  if (!(a > T(-1) && a < T(W) && b > T(-1) && b < T(H))) return T(0);
+  const T fa = std::floor(a), fb = std::floor(b);
+  const int i = (int)fa, j = (int)fb;
+  const T al = a - fa, be = b - fb;
+  auto at = [&](int w, int h) { return (w < 0 || w >= W || h < 0 || h >= H) ? T(0) : f(w, h); };
+  return (T(1) - be) * ((T(1) - al) * at(i, j) + al * at(i + 1, j)) +
+         be * ((T(1) - al) * at(i, j + 1) + al * at(i + 1, j + 1));
+}
+
+// DD3_GPU.cu:673-754 (kernel), :756-834 (host: vol *= mask; border-addressed linear
+// texture of width ZN, height XN, depth YN).  One ray from the source to each detector
+// cell centre, sampled where it crosses the centre plane of every x-slice (|cos| <=
+// |sin|) or y-slice, times the path length per slice dx / |dir_x| (or / |dir_y|).
+template <typename T>
+void fp_pdd(const Geo<T>& g, const float* vol, T* prj) {
+  const int XN = g.XN, ZN = g.ZN;
+  auto voxel = [&](int x, int y, int z) {
+    return T(vol[((size_t)y * XN + x) * ZN + z]) * T(g.mask[(size_t)y * XN + x]);
+  };
+#pragma omp parallel for collapse(2) schedule(dynamic, 4)
+  for (int p = 0; p < g.PN; ++p)
+    for (int u = 0; u < g.DNU; ++u)
+      for (int v = 0; v < g.DNV; ++v) {
+        const T c = g.cs[p], s = g.sn[p];
+        const T sx = g.x0 * c - g.y0 * s, sy = g.x0 * s + g.y0 * c, sz = g.z0 + g.zsh[p];
+        const T Dx = g.xds[u] * c - g.yds[u] * s, Dy = g.xds[u] * s + g.yds[u] * c;
+        const T Dz = g.zds[v] + g.zsh[p];
+        T ex = Dx - sx, ey = Dy - sy, ez = Dz - sz;
+        const T n = std::sqrt(ex * ex + ey * ey + ez * ez);
+        ex /= n; ey /= n; ez /= n;
+        T summ = 0, out;
+        if (std::fabs(c) <= std::fabs(s)) {
+          for (int ii = 0; ii < XN; ++ii) {
+            const T obj = (T(ii) - g.ocx) * g.dx;
+            const T yy = (obj - Dx) / ex * ey + Dy, zz = (obj - Dx) / ex * ez + Dz;
+            summ += lerp2_border([&](int w, int h) { return voxel(ii, h, w); }, ZN, g.YN,
+                                 zz / g.dz + g.ocz, yy / g.dx + g.ocy);
+          }
+          out = summ * g.dx / std::fabs(ex);
+        } else {
+          for (int jj = 0; jj < g.YN; ++jj) {
+            const T obj = (T(jj) - g.ocy) * g.dx;
+            const T xx = (obj - Dy) / ey * ex + Dx, zz = (obj - Dy) / ey * ez + Dz;
+            summ += lerp2_border([&](int w, int h) { return voxel(h, jj, w); }, ZN, XN,
+                                 zz / g.dz + g.ocz, xx / g.dx + g.ocx);
+          }
+          out = summ * g.dx / std::fabs(ey);
+        }
+        prj[((size_t)p * g.DNU + u) * g.DNV + v] = out;
+      }
+}
+
+// DD3_GPU.cu:1478-1528 (kernel), host :1664-1700 (border-addressed linear texture of
+// width DNV, height DNU, depth PN; constants of calDetParas).  Voxel driven: each view's
+// sinogram is sampled where the ray from the source through the voxel centre meets the
+// detector, weighted by dx |dir| / |dir_x| (or / |dir_y|, chosen by the larger component
+// of the source position).  Voxels whose mask byte is not 1 are 0.
+template <typename T>
+void bp_pdd(const Geo<T>& g, const float* prj, T* vol) {
+  const BackConsts<T> k = back_consts(g);
+  const int DNU = g.DNU, DNV = g.DNV;
+#pragma omp parallel for collapse(2) schedule(dynamic, 4)
+  for (int y = 0; y < g.YN; ++y)
+    for (int x = 0; x < g.XN; ++x) {
+      T* col = vol + ((size_t)y * g.XN + x) * g.ZN;
+      if (g.mask[y * g.XN + x] != 1) {
+        for (int z = 0; z < g.ZN; ++z) col[z] = T(0);
+        continue;
+      }
+      const T Px = (T(x) - g.ocx) * g.dx, Py = (T(y) - g.ocy) * g.dx;
+      for (int z = 0; z < g.ZN; ++z) {
+        const T Pz = (T(z) - g.ocz) * g.dz;
+        T summ = 0;
+        for (int p = 0; p < g.PN; ++p) {
+          const T c = g.cs[p], s = g.sn[p];
+          const T sx = g.x0 * c - g.y0 * s, sy = g.x0 * s + g.y0 * c, sz = g.z0 + g.zsh[p];
+          const T dX = Px - sx, dY = Py - sy, dZ = Pz - sz;
+          const T ddv = dX * dX + dY * dY;
+          const T invl = T(1) / std::sqrt(ddv);
+          const T iz = dZ * k.S * invl + k.cV;
+          const T alpha = std::asin((sy * dX - sx * dY) * invl * k.inv_sid);
+          T cosVal = (std::fabs(sx) >= std::fabs(sy)) ? std::fabs(T(1) / dX) : std::fabs(T(1) / dY);
+          cosVal *= g.dx * std::sqrt(ddv + dZ * dZ);
+          const T iu = alpha * k.ib + k.cU;
+          const float* plane = prj + (size_t)p * DNU * DNV;
+          summ += lerp2_border([&](int w, int h) { return T(plane[(size_t)h * DNV + w]); }, DNV,
+                               DNU, iz, iu) *
+                  cosVal;
+        }
+        col[z] = summ;
+      }
+    }
+}
+
+// ---------------- FP64 branchless (prjMode 2 forward) ----------------
+// Point-sampled integral image with clamp addressing and explicit bilinear weights
+// between floor and ceil of the coordinates (DD3_GPU.cu:369-392, bilerp of
+// utilities.cuh:245-254): first along H at both W rows, then along W.
+template <typename T>
+inline T sat_point(const T* pl, int W, int H, T a, T b) {
+  a = std::min(std::max(a, T(-1)), T(W));
+  b = std::min(std::max(b, T(-1)), T(H));
+  const T fa = std::floor(a), ca = std::ceil(a), fb = std::floor(b), cb = std::ceil(b);
+  auto at = [&](T w, T h) {
+    const int wi = std::min(std::max((int)w, 0), W - 1), hi = std::min(std::max((int)h, 0), H - 1);
+    return pl[(size_t)hi * W + wi];
+  };
+  const T t1 = b - fb, t2 = a - fa;
+  const T vv0 = at(fa, fb) * (T(1) - t1) + at(fa, cb) * t1;
+  const T vv1 = at(ca, fb) * (T(1) - t1) + at(ca, cb) * t1;
+  return vv0 * (T(1) - t2) + vv1 * t2;
+}
+
+// DD3_GPU.cu:255-466 (kernel), :468-660 (host): the branchless footprint model with the
+// texel coordinates real / d + objCntIdx + 1 sampled without the half-texel offset of
+// linear filtering.
+template <typename T>
+void fp_dp(const Geo<T>& g, const float* vol, T* prj) {
+  std::vector<T> zxy, zyx;
+  volume_sats(g, vol, zxy, zyx);
+  const int W = g.ZN + 1;
+#pragma omp parallel for collapse(2) schedule(dynamic, 4)
+  for (int p = 0; p < g.PN; ++p)
+    for (int u = 0; u < g.DNU; ++u)
+      for (int v = 0; v < g.DNV; ++v) {
+        Ray<T> r = ray_setup(g, p, u, v);
+        T summ = 0;
+        const int NS = r.xmarch ? g.XN : g.YN;
+        const int H = r.xmarch ? g.YN + 1 : g.XN + 1;
+        const T* base = r.xmarch ? zyx.data() : zxy.data();
+        for (int ii = 0; ii < NS; ++ii) {
+          T tL, tR, tD, tU, lenL, lenH;
+          footprint(g, r, ii, tL, tR, tD, tU, lenL, lenH);
+          const T* pl = base + (size_t)ii * H * W;
+          const T a = sat_point(pl, W, H, tD + 1, tL + 1) + sat_point(pl, W, H, tU + 1, tR + 1) -
+                      sat_point(pl, W, H, tD + 1, tR + 1) - sat_point(pl, W, H, tU + 1, tL + 1);
+          summ += a / (lenL * lenH);
+        }
+        prj[((size_t)p * g.DNU + u) * g.DNV + v] = summ * r.constVal;
+      }
+}
+
 // Geometry restricted to the views s, s+osNum, s+2*osNum, ... (os_sart.m:15/23,
 // zero-based subset index s).
 template <typename T>
@@ -658,6 +809,21 @@ void orc_bp_direct_f64(ORC_GEO_ARGS, double* vol, const float* prj, float dx, fl
 void orc_bp_adjoint_f64(ORC_GEO_ARGS, double* vol, const float* prj, float dx, float dz,
                         const uint8_t* mask) {
   bp_adjoint(make_geo<double>(ORC_GEO_PASS, dx, dz, mask), prj, vol);
+}
+
+void orc_fp_pdd_f64(ORC_GEO_ARGS, const float* vol, double* prj, float dx, float dz,
+                    const uint8_t* mask) {
+  fp_pdd(make_geo<double>(ORC_GEO_PASS, dx, dz, mask), vol, prj);
+}
+
+void orc_bp_pdd_f64(ORC_GEO_ARGS, double* vol, const float* prj, float dx, float dz,
+                    const uint8_t* mask) {
+  bp_pdd(make_geo<double>(ORC_GEO_PASS, dx, dz, mask), prj, vol);
+}
+
+void orc_fp_dp_f64(ORC_GEO_ARGS, const float* vol, double* prj, float dx, float dz,
+                   const uint8_t* mask) {
+  fp_dp(make_geo<double>(ORC_GEO_PASS, dx, dz, mask), vol, prj);
 }
 
 // FP32 instantiation of the SAT transcription: the CPU baseline.

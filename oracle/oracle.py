@@ -58,13 +58,15 @@ def _mask(g):
 
 def fp(g, vol, method="sat"):
     """FP64 forward projection of float32 vol[y, x, z] -> float64 prj[p, u, v].
-    method: 'sat' (transcription of DD3_GPU.cu:844-990) or 'direct'."""
+    method: 'sat' (transcription of DD3_GPU.cu:844-990), 'direct', 'pdd' (pseudo distance
+    driven, DD3_GPU.cu:673-834) or 'dp' (FP64 branchless, DD3_GPU.cu:255-660)."""
     vol = np.ascontiguousarray(vol, dtype=np.float32)
     assert vol.shape == g.vol_shape
     out = np.zeros(g.prj_shape, dtype=np.float64)
     args, keep = _geo_args(g)
     m = _mask(g)
-    fn = {"sat": lib().orc_fp_sat_f64, "direct": lib().orc_fp_direct_f64}[method]
+    fn = {"sat": lib().orc_fp_sat_f64, "direct": lib().orc_fp_direct_f64,
+          "pdd": lib().orc_fp_pdd_f64, "dp": lib().orc_fp_dp_f64}[method]
     fn(*args, _ptr(vol), _ptr(out), _f(g.dx), _f(g.dz), _ptr(m))
     return out
 
@@ -82,14 +84,15 @@ def fp64(g, vol64):
 def bp(g, prj, method="sat"):
     """FP64 back projection of float32 prj[p, u, v] -> float64 vol[y, x, z].
     method: 'sat' / 'direct' (reference arithmetic, DD3_GPU.cu:1397-1475) or
-    'adjoint' (exact transpose of the forward projector)."""
+    'adjoint' (exact transpose of the forward projector) or 'pdd' (pseudo distance
+    driven, DD3_GPU.cu:1478-1528)."""
     prj = np.ascontiguousarray(prj, dtype=np.float32)
     assert prj.shape == g.prj_shape
     out = np.zeros(g.vol_shape, dtype=np.float64)
     args, keep = _geo_args(g)
     m = _mask(g)
     fn = {"sat": lib().orc_bp_sat_f64, "direct": lib().orc_bp_direct_f64,
-          "adjoint": lib().orc_bp_adjoint_f64}[method]
+          "adjoint": lib().orc_bp_adjoint_f64, "pdd": lib().orc_bp_pdd_f64}[method]
     fn(*args, _ptr(out), _ptr(prj), _f(g.dx), _f(g.dz), _ptr(m))
     return out
 

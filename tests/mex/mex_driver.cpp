@@ -129,15 +129,16 @@ int main(int argc, char** argv) {
       c.rhs.push_back(mxCreateString("Spin"));
       std::cout << "action " << run(c, 1, out) << "\n";
     }
-    {  // unimplemented projector mode (pseudo-DD)
+    {  // projector mode without a matched adjoint (pseudo-DD back, bit 4 set)
       Call c;
-      geometry(c, "Proj", "");
+      geometry(c, "Back", "");
       c.rhs.push_back(vecf(std::vector<float>(8, 0.f)));
       c.rhs.push_back(f32(1));
       c.rhs.push_back(f32(1));
       c.rhs.push_back(veci8({1, 1, 1, 1}));
       c.rhs.push_back(f64(0));
-      c.rhs.push_back(i32(1));
+      c.rhs.push_back(i32(0));
+      c.rhs.push_back(i32(1 | 16));
       std::cout << "mode " << run(c, 1, out) << "\n";
     }
     return 0;

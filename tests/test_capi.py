@@ -76,26 +76,27 @@ def test_plan_create_rejects_bad_geometry():
     assert _create(cg)[0] == _lib.CTR_ERR_ARG
 
 
-def test_reference_entry_points_reject_unimplemented_modes():
-    """prjMode 1-3 (pseudo-DD, FP64 texture, z-line) are not built: a clean error, not a
-    silent fall-back to another model."""
+def test_reference_entry_points_argument_checks():
+    """Checked before any device work: the matched-adjoint bit is refused with the
+    pseudo-DD back projector (it exists for the branchless model only), and NULL data
+    buffers are argument errors."""
     g = G.dd3_geometry(G.config_small())
     z = np.zeros(4, np.float32)
-    for mode in (1, 2, 3):
-        rc = lib().ctr_dd3_proj(g.x0, g.y0, g.z0, g.DNU, g.DNV, z.ctypes.data, z.ctypes.data,
-                                z.ctypes.data, 0, 0, 0, z.ctypes.data, z.ctypes.data, g.PN,
-                                g.XN, g.YN, g.ZN, z.ctypes.data, z.ctypes.data, g.dx, g.dz,
-                                None, 0, mode)
-        assert rc == _lib.CTR_ERR_UNSUPPORTED
+    for mode in (1 | 16, 2 | 16):
         rc = lib().ctr_dd3_back(g.x0, g.y0, g.z0, g.DNU, g.DNV, z.ctypes.data, z.ctypes.data,
                                 z.ctypes.data, 0, 0, 0, z.ctypes.data, z.ctypes.data, g.PN,
                                 g.XN, g.YN, g.ZN, z.ctypes.data, z.ctypes.data, g.dx, g.dz,
                                 None, 0, 0, mode)
         assert rc == _lib.CTR_ERR_UNSUPPORTED
-    rc = lib().ctr_dd3_proj(g.x0, g.y0, g.z0, g.DNU, g.DNV, z.ctypes.data, z.ctypes.data,
-                            z.ctypes.data, 0, 0, 0, z.ctypes.data, z.ctypes.data, g.PN, g.XN,
-                            g.YN, g.ZN, None, None, g.dx, g.dz, None, 0, 0)
-    assert rc == _lib.CTR_ERR_ARG
+    for mode in (0, 1, 2, 3):
+        rc = lib().ctr_dd3_proj(g.x0, g.y0, g.z0, g.DNU, g.DNV, z.ctypes.data, z.ctypes.data,
+                                z.ctypes.data, 0, 0, 0, z.ctypes.data, z.ctypes.data, g.PN,
+                                g.XN, g.YN, g.ZN, None, None, g.dx, g.dz, None, 0, mode)
+        assert rc == _lib.CTR_ERR_ARG
+        rc = lib().ctr_dd3_back(g.x0, g.y0, g.z0, g.DNU, g.DNV, z.ctypes.data, z.ctypes.data,
+                                z.ctypes.data, 0, 0, 0, z.ctypes.data, z.ctypes.data, g.PN,
+                                g.XN, g.YN, g.ZN, None, None, g.dx, g.dz, None, 0, 0, mode)
+        assert rc == _lib.CTR_ERR_ARG
 
 
 def test_null_plan_calls_fail_cleanly():

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from conftest import rel_l2
-from cases import CASES, clinical, sinogram, small, volume
+from cases import CASES, clinical, sinogram, small, tiny, volume
 
 pytestmark = pytest.mark.gpu
 
@@ -156,13 +156,22 @@ def test_dd3_matlab_surface(gpu_plan_factory):
 
 
 def test_reference_mode_errors(gpu_plan_factory):
+    """The matched adjoint (bit 4) exists for the branchless model only: prjMode 1 | 16
+    is refused; 0 | 16 runs."""
     from ctrecon_amd import dd3 as D
-    from ctrecon_amd._lib import CtrError
+    from ctrecon_amd._lib import CtrError, lib
 
-    cfg, g = small()
-    img = np.zeros((g.ZN, g.XN, g.YN), np.float32)
-    with pytest.raises(CtrError):
-        D.dd3("fp_gpu_pseudo_dd", cfg, img)
+    cfg, g = tiny()
+    argv, keep = D._geo_argv(g)
+    m = np.ascontiguousarray(g.mask, dtype=np.uint8)
+    vol = np.empty(g.XN * g.YN * g.ZN, np.float32)
+    prj = np.ones(g.PN * g.DNU * g.DNV, np.float32)
+    rc = lib().ctr_dd3_back(*argv, vol.ctypes.data, prj.ctypes.data, g.dx, g.dz, m.ctypes.data,
+                            0, 0, 1 | 16)
+    assert rc == 3
+    rc = lib().ctr_dd3_back(*argv, vol.ctypes.data, prj.ctypes.data, g.dx, g.dz, m.ctypes.data,
+                            0, 0, 0 | 16)
+    assert rc == 0
 
 
 def test_clinical_sampled_views(oracle, gpu_plan_factory):

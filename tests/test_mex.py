@@ -15,7 +15,10 @@ DRIVER = os.path.join(HERE, "mex", "_build", "mex_driver")
 
 
 def _driver():
-    if not os.path.exists(DRIVER):
+    srcs = [os.path.join(HERE, "mex", "mex_driver.cpp"), os.path.join(HERE, "mex", "mex.h"),
+            os.path.join(os.path.dirname(HERE), "ctrecon_amd", "mex", "DD3_GPU_mex.cpp")]
+    if not os.path.exists(DRIVER) or any(os.path.getmtime(s) > os.path.getmtime(DRIVER)
+                                         for s in srcs):
         subprocess.run(["bash", os.path.join(HERE, "mex", "build.sh")], check=True)
     return DRIVER
 
@@ -27,7 +30,7 @@ def test_gateway_errors():
     assert lines["arity"].startswith("DD3_GPU:usage") and "24" in lines["arity"]
     assert lines["class"].startswith("DD3_GPU:type")
     assert lines["action"].startswith("DD3_GPU:usage")
-    assert lines["mode"].startswith("DD3_GPU:ctr") and "not implemented" in lines["mode"]
+    assert lines["mode"].startswith("DD3_GPU:ctr") and "matched adjoint" in lines["mode"]
 
 
 @pytest.mark.gpu
