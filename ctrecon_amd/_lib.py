@@ -23,6 +23,7 @@ CTR_BP_MATCHED = 1
 CTR_METHOD_DIRECT = 0
 CTR_METHOD_SAT = 1
 CTR_METHOD_PSEUDO_DD = 2
+CTR_METHOD_ZLINE = 3
 CTR_STORAGE_F32 = 0
 CTR_STORAGE_BF16 = 1
 CTR_STORAGE_BF16_SINO = 2

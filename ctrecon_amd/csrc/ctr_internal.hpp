@@ -41,6 +41,7 @@ struct BpConsts {
 // on detectors several hundred cells wide.
 struct BpConsts64 {
   double ib, cU, inv_sid, dx, ocx, ocy;
+  double S, cV, ocz;  // S2D / ddv, detCtrIdxV, objCntIdx.z (the z-line back projector)
 };
 
 // Geometry constants every kernel needs.
@@ -132,6 +133,7 @@ struct Plan {
   float* h2 = nullptr;         // [DNV]        (zds_v - z0)^2
   float* invdB = nullptr;      // [DNV]        1/(Bz[v+1]-Bz[v])
   double2* rot64 = nullptr;    // [PN]         (cos, sin) of the regularised view angle, FP64
+  double* sz64 = nullptr;      // [PN]         source z (z0 + zshift), FP64
   double2* det64 = nullptr;    // [DNU]        detector cell centres (xds, yds), FP64
   float* zc = nullptr;         // [DNV]        (zds_v - z0)/dz
   uint8_t* mask = nullptr;     // [XN*YN]
@@ -153,6 +155,8 @@ struct Plan {
   float* scratch_prj = nullptr;     // [max_views][DNU][DNV]
   size_t scratch_prj_views = 0;
   double* partials = nullptr;       // reduction partials
+  double* zl_cpre = nullptr;        // z-line back projector: row prefix of one launch's views
+  size_t zl_cpre_n = 0;
   size_t partials_n = 0;
 };
 
@@ -215,6 +219,10 @@ int launch_fp_pdd(const Plan& P, const float* vol, float* out, int vb, int vs, i
                   hipStream_t st);
 int launch_bp_pdd(const Plan& P, const float* prj, float* vol, int vb, int vs, int vc,
                   hipStream_t st);
+// z-line back projector (pdd.hip); cpre: zline_prefix_doubles(d, vc) doubles of scratch
+size_t zline_prefix_doubles(const Dims& d, int vc);
+int launch_bp_zline(const Plan& P, const float* prj, double* cpre, float* vol, int vb, int vs,
+                    int vc, hipStream_t st);
 // integral-image method (sat.hip)
 size_t sat_volume_floats(const Dims& d);
 size_t sat_sino_floats(const Dims& d, int vc);

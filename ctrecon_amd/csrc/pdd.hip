@@ -1,6 +1,6 @@
-// gfx950 kernels of the reference's pseudo distance-driven projector pair
-// (iterativeRecon/DD3_GPU.cu:673-834 forward, :1478-1528 back; prjMode 1 / 3 forward,
-// 1 / 2 back).
+// gfx950 kernels of the reference's other projector models: the pseudo distance-driven
+// pair (iterativeRecon/DD3_GPU.cu:673-834 forward, :1478-1528 back; prjMode 1 / 3
+// forward, 1 / 2 back) and the z-line back projector (:1545-1647, prjMode 3 back).
 //
 // The reference samples linear-filtered, border-addressed textures (zero outside the
 // array).  Here every sample is an explicit bilinear fetch of the array with exact FP32
@@ -239,6 +239,161 @@ __global__ __launch_bounds__(kBlock) void bp_pdd_kernel(
     if (zb + 64 * g < d.ZN) vcol[zb + 64 * g] = acc[g];
 }
 
+// ---------------------------------------------------------------------------
+// z-line back projector (DD3_GPU.cu:1545-1647), prjMode 3.
+//
+// The reference accumulates, per voxel column and block of 16 slices, 17 face terms
+//   T_p(j) = cosVal_p w_p(j) [SAT_p(tD_j, tR) - SAT_p(tD_j, tL)]
+// over the views (face j: the lower face of slice j; w_p(j) = hypot(d_z, |d_xy|)/|d_xy|
+// at the centre of slice j; [tL, tR]: fan-angle range of the column's extent across the
+// rays), and writes slice k = S(k+1) - S(k), S(j) = sum_p T_p(j), for the slices of the
+// whole blocks (k < 16 * floor(ZN / 16)); the others stay 0.
+//
+// Here a wave owns one column and its lanes the faces; S is accumulated in FP64 (the
+// slice values are differences of sums that carry whole detector columns) and the
+// bracket is the exact integral of the piecewise-constant view over the rows below tD
+// and the columns of [tL, tR]: per column u, C_u(m) + frac * prj[u][m], C_u the
+// exclusive FP64 prefix over the rows (zl_prefix_kernel).  The upper face of a wave's
+// last slice is one more face, taken by lane 0 of an extra group.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void zl_prefix_kernel(const float* __restrict__ prj,
+                                                           double* __restrict__ cpre, int rows,
+                                                           int DNV) {
+  const int row = blockIdx.x * kWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (row >= rows) return;
+  const int lane = threadIdx.x & 63;
+  const size_t base = (size_t)row * DNV;
+  double carry = 0.0;
+  for (int c0 = 0; c0 < DNV; c0 += 64) {
+    const int v = c0 + lane;
+    const double x = v < DNV ? (double)prj[base + v] : 0.0;
+    double inc = x;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const double t = __shfl_up(inc, off, 64);
+      if (lane >= off) inc += t;
+    }
+    if (v < DNV) cpre[base + v] = carry + inc - x;
+    carry += __shfl(inc, 63, 64);
+  }
+}
+
+template <int ZG>
+__global__ __launch_bounds__(kBlock) void bp_zline_kernel(
+    const float* __restrict__ prj, const double* __restrict__ cpre, float* __restrict__ vol,
+    const ViewP* __restrict__ views, const double2* __restrict__ src64,
+    const double* __restrict__ sz64, const uint8_t* __restrict__ mask, Dims d, BpConsts64 c64,
+    int vb, int vs, int vc, int NZ) {
+  const int lane = threadIdx.x & 63;
+  const int col = blockIdx.x * kWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (col >= d.XN * d.YN) return;
+  const int zb0 = blockIdx.y * (64 * ZG);
+  const int y = col / d.XN, x = col - y * d.XN;
+  float* vcol = vol + (size_t)col * d.ZN;
+  if (mask[col] != 1 || zb0 >= NZ) {
+#pragma unroll
+    for (int g = 0; g < ZG; ++g)
+      if (zb0 + 64 * g + lane < d.ZN) vcol[zb0 + 64 * g + lane] = 0.f;
+    return;
+  }
+  const double Px = ((double)x - c64.ocx) * c64.dx;
+  const double Py = ((double)y - c64.ocy) * c64.dx;
+  const double hdx = 0.5 * c64.dx, dz = d.dz, ocz = c64.ocz, cV = c64.cV;
+  // faces of the wave: zb0 .. ftop (ftop <= NZ)
+  const int ftop = min(zb0 + 64 * ZG, NZ);
+  const double DNVd = (double)d.DNV;
+  double S[ZG + 1];
+#pragma unroll
+  for (int g = 0; g <= ZG; ++g) S[g] = 0.0;
+  for (int k0 = 0; k0 < vc; k0 += 64) {
+    const int kl = k0 + lane;
+    double q_s2v = 0.0, q_sz = 0.0, q_ang = 0.0, q_cw = 0.0, q_wu0 = 0.0, q_wul = 0.0;
+    int q_u0 = 0, q_nu = 0;
+    bool vis = false;
+    if (kl < vc) {
+      const int p = vb + kl * vs;
+      const ViewP vp = views[p];
+      const double2 s = src64[p];
+      const double dX = Px - s.x, dY = Py - s.y;
+      const double s2v = sqrt(dX * dX + dY * dY);
+      double lx, ly, rx, ry, cosv;
+      if (vp.xmarch) {
+        lx = dX; ly = dY - hdx; rx = dX; ry = dY + hdx;
+        cosv = c64.dx * s2v / dX;
+      } else {
+        lx = dX + hdx; ly = dY; rx = dX - hdx; ry = dY;
+        cosv = c64.dx * s2v / dY;
+      }
+      const double tL =
+          asin_fan((s.y * lx - s.x * ly) * rsqrt(lx * lx + ly * ly) * c64.inv_sid) * c64.ib +
+          c64.cU;
+      const double tR =
+          asin_fan((s.y * rx - s.x * ry) * rsqrt(rx * rx + ry * ry) * c64.inv_sid) * c64.ib +
+          c64.cU;
+      const double lo = fmin(fmax(fmin(tL, tR), -0.5), d.DNU - 0.5);
+      const double hi = fmin(fmax(fmax(tL, tR), -0.5), d.DNU - 0.5);
+      const double ang = c64.S / s2v;
+      const double sz = sz64[p];
+      // the rows below the wave's highest face must not all lie below the detector
+      const double ta = (((double)zb0 - ocz) * dz - sz - 0.5 * dz) * ang + cV;
+      const double tb = (((double)ftop - ocz) * dz - sz - 0.5 * dz) * ang + cV;
+      vis = lo < hi && fmax(ta, tb) > -0.5;
+      if (vis) {
+        const int u0 = (int)floor(lo + 0.5);
+        const int nu = min((int)floor(hi + 0.5), d.DNU - 1) - u0 + 1;
+        q_wu0 = (nu == 1) ? hi - lo : ((double)u0 + 0.5) - lo;
+        q_wul = hi - ((double)(u0 + nu - 1) - 0.5);
+        q_u0 = u0;
+        q_nu = nu;
+        q_cw = (tR >= tL) ? cosv : -cosv;
+        q_s2v = s2v;
+        q_sz = sz;
+        q_ang = ang;
+      }
+    }
+    uint64_t live = __ballot(vis);
+    while (live) {
+      const int j = __builtin_ctzll(live);
+      live &= live - 1;
+      const int k = k0 + j;
+      const double s2v = __shfl(q_s2v, j, 64), sz = __shfl(q_sz, j, 64);
+      const double ang = __shfl(q_ang, j, 64), cw = __shfl(q_cw, j, 64);
+      const double wu0 = __shfl(q_wu0, j, 64), wul = __shfl(q_wul, j, 64);
+      const int u0 = __builtin_amdgcn_readlane(q_u0, j), nu = __builtin_amdgcn_readlane(q_nu, j);
+      const float* pk = prj + (size_t)k * d.DNU * d.DNV;
+      const double* ck = cpre + (size_t)k * d.DNU * d.DNV;
+#pragma unroll
+      for (int g = 0; g <= ZG; ++g) {
+        const int f = zb0 + 64 * g + lane;
+        const bool fok = (g < ZG) ? f <= NZ : (lane == 0 && f <= NZ);
+        if (!__any(fok)) continue;
+        if (!fok) continue;
+        const double dirZ = ((double)f - ocz) * dz - sz;
+        const double tD = (dirZ - 0.5 * dz) * ang + cV;
+        if (!(tD > -0.5)) continue;
+        const double tt = fmin(tD + 0.5, DNVd);
+        const int m = min((int)tt, d.DNV - 1);
+        const double fr = tt - (double)m;
+        double G = 0.0;
+        for (int iu = 0; iu < nu; ++iu) {
+          const double wu = (iu == 0) ? wu0 : (iu == nu - 1 ? wul : 1.0);
+          const size_t o = (size_t)(u0 + iu) * d.DNV + m;
+          G = fma(wu, fma(fr, (double)pk[o], ck[o]), G);
+        }
+        S[g] = fma(cw * sqrt(fma(dirZ, dirZ, s2v * s2v)) / s2v, G, S[g]);
+      }
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < ZG; ++g) {
+    const int z = zb0 + 64 * g + lane;
+    double up = __shfl_down(S[g], 1, 64);
+    const double first_next = __shfl(S[g + 1], 0, 64);
+    if (lane == 63) up = first_next;
+    if (z < d.ZN) vcol[z] = (z < NZ) ? (float)(up - S[g]) : 0.f;
+  }
+}
+
 int check(const char* what) {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(CTR_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
@@ -277,6 +432,34 @@ int launch_bp_pdd(const Plan& P, const float* prj, float* vol, int vb, int vs, i
   }
 #undef CTR_BPP
   return check("bp_pdd_kernel");
+}
+
+size_t zline_prefix_doubles(const Dims& d, int vc) { return (size_t)vc * d.DNU * d.DNV; }
+
+int launch_bp_zline(const Plan& P, const float* prj, double* cpre, float* vol, int vb, int vs,
+                    int vc, hipStream_t st) {
+  const Dims& d = P.d;
+  const int rows = vc * d.DNU;
+  if (rows > 0) {
+    hipLaunchKernelGGL(zl_prefix_kernel, dim3((rows + kWaves - 1) / kWaves), dim3(kBlock), 0, st,
+                       prj, cpre, rows, d.DNV);
+    int rc = check("zl_prefix_kernel");
+    if (rc) return rc;
+  }
+  const int NZ = 16 * (d.ZN / 16);
+  const int zg = std::min(4, (d.ZN + 63) / 64);
+  dim3 grid((d.XN * d.YN + kWaves - 1) / kWaves, (d.ZN + 64 * zg - 1) / (64 * zg));
+#define CTR_BPZ(ZGV)                                                                          \
+  hipLaunchKernelGGL(bp_zline_kernel<ZGV>, grid, dim3(kBlock), 0, st, prj, cpre, vol, P.views, \
+                     P.src64, P.sz64, P.mask, d, P.bc64, vb, vs, vc, NZ)
+  switch (zg) {
+    case 1: CTR_BPZ(1); break;
+    case 2: CTR_BPZ(2); break;
+    case 3: CTR_BPZ(3); break;
+    default: CTR_BPZ(4); break;
+  }
+#undef CTR_BPZ
+  return check("bp_zline_kernel");
 }
 
 }  // namespace ctr

@@ -158,11 +158,15 @@ static int build_tables(const ctr_geometry* g, Plan& P, double half = 0.0) {
     P.bc64.dx = dx;
     P.bc64.ocx = ocx;
     P.bc64.ocy = ocy;
+    P.bc64.S = S2D / ddv;
+    P.bc64.cV = cV;
+    P.bc64.ocz = ocz;
   }
 
   // per-view tables
   std::vector<ViewP> views(PN);
   std::vector<double2> src64(PN), rot64(PN), det64(DNU);
+  std::vector<double> sz64(PN);
   std::vector<float> zc(DNV);
   for (int u = 0; u < DNU; ++u) det64[u] = make_double2(g->xds[u], g->yds[u]);
   for (int v = 0; v < DNV; ++v) zc[v] = (float)((g->zds[v] - z0) / dz);
@@ -176,6 +180,7 @@ static int build_tables(const ctr_geometry* g, Plan& P, double half = 0.0) {
     vp.sx = (float)sx; vp.sy = (float)sy; vp.sz = (float)sz;
     src64[p] = make_double2(sx, sy);
     rot64[p] = make_double2(c, s);
+    sz64[p] = sz;
     vp.az = (float)(sz / dz + ocz + half);
     vp.xmarch = std::fabs(c) <= std::fabs(s) ? 1 : 0;  // DD3_GPU.cu:914
     vp.wx = std::fabs(sx) > std::fabs(sy) ? 1 : 0;  // DD3_GPU.cu:1453
@@ -256,6 +261,7 @@ static int build_tables(const ctr_geometry* g, Plan& P, double half = 0.0) {
   if ((rc = upload(&P.views, views))) return rc;
   if ((rc = upload(&P.src64, src64))) return rc;
   if ((rc = upload(&P.rot64, rot64))) return rc;
+  if ((rc = upload(&P.sz64, sz64))) return rc;
   if ((rc = upload(&P.det64, det64))) return rc;
   if ((rc = upload(&P.zc, zc))) return rc;
   if ((rc = upload(&P.bnd, bnd))) return rc;
@@ -272,6 +278,8 @@ static void free_plan(Plan* P) {
   (void)hipFree(P->views);
   (void)hipFree(P->src64);
   (void)hipFree(P->rot64);
+  (void)hipFree(P->sz64);
+  (void)hipFree(P->zl_cpre);
   (void)hipFree(P->det64);
   (void)hipFree(P->zc);
   (void)hipFree(P->bnd);
@@ -335,7 +343,7 @@ static int ensure_buf(float** p, size_t* have, size_t need, const char* what) {
 template <typename TV, typename TO = TV>
 static int fp_any(Plan& P, const TV* vol, TO* out, int vb, int vs, int vc, hipStream_t st,
                   const float* b, const float* invRow, double* partials) {
-  if (P.method == CTR_METHOD_PSEUDO_DD) {
+  if (P.method == CTR_METHOD_PSEUDO_DD || P.method == CTR_METHOD_ZLINE) {
     if constexpr (std::is_same<TV, float>::value && std::is_same<TO, float>::value) {
       if (b) return fail(CTR_ERR_UNSUPPORTED, "the pseudo-DD projector has no fused residual");
       return launch_fp_pdd(P, vol, out, vb, vs, vc, st);
@@ -358,13 +366,24 @@ template <typename T>
 static int bp_any(Plan& P, const T* prj, float* vol, int vb, int vs, int vc, int kind,
                   hipStream_t st, const float* invCol, int accumulate,
                   uint16_t* shadow = nullptr) {
-  if (P.method == CTR_METHOD_PSEUDO_DD) {
+  if (P.method == CTR_METHOD_PSEUDO_DD || P.method == CTR_METHOD_ZLINE) {
     if (kind != CTR_BP_REFERENCE)
-      return fail(CTR_ERR_UNSUPPORTED, "the pseudo-DD pair has no matched adjoint");
+      return fail(CTR_ERR_UNSUPPORTED, "the pseudo-DD / z-line models have no matched adjoint");
     if (accumulate)
-      return fail(CTR_ERR_UNSUPPORTED, "the pseudo-DD back projector has no update form");
+      return fail(CTR_ERR_UNSUPPORTED, "the pseudo-DD / z-line back projectors have no update form");
     if constexpr (std::is_same<T, float>::value) {
-      return launch_bp_pdd(P, prj, vol, vb, vs, vc, st);
+      if (P.method == CTR_METHOD_PSEUDO_DD) return launch_bp_pdd(P, prj, vol, vb, vs, vc, st);
+      const size_t need = zline_prefix_doubles(P.d, vc);
+      if (P.zl_cpre_n < need) {
+        (void)hipFree(P.zl_cpre);
+        P.zl_cpre = nullptr;
+        P.zl_cpre_n = 0;
+        if (hipMalloc((void**)&P.zl_cpre, std::max<size_t>(need, 1) * sizeof(double)) !=
+            hipSuccess)
+          return fail(CTR_ERR_NOMEM, "cannot allocate the z-line row prefix");
+        P.zl_cpre_n = need;
+      }
+      return launch_bp_zline(P, prj, P.zl_cpre, vol, vb, vs, vc, st);
     } else {
       return fail(CTR_ERR_UNSUPPORTED, "the pseudo-DD projector takes FP32 operands");
     }
@@ -388,7 +407,7 @@ static int bp_any(Plan& P, const T* prj, float* vol, int vb, int vs, int vc, int
 // The OS-SART and CG drivers run the reference's mode-0 model (os_sart.m:16-27 and
 // conjgrad.m call the branchless projectors only).
 static int check_solver_method(const Plan& P) {
-  if (P.method == CTR_METHOD_PSEUDO_DD)
+  if (P.method == CTR_METHOD_PSEUDO_DD || P.method == CTR_METHOD_ZLINE)
     return fail(CTR_ERR_UNSUPPORTED,
                 "the OS-SART / CG drivers use the branchless model (method DIRECT or SAT)");
   return CTR_OK;
@@ -529,9 +548,10 @@ int ctr_plan_destroy(ctr_plan* plan) {
 int ctr_plan_set_method(ctr_plan* plan, int method) {
   Plan* P = get(plan);
   if (!P) return fail(CTR_ERR_ARG, "NULL plan");
-  if (method != CTR_METHOD_DIRECT && method != CTR_METHOD_SAT && method != CTR_METHOD_PSEUDO_DD)
-    return fail(CTR_ERR_ARG,
-                "method must be CTR_METHOD_DIRECT, CTR_METHOD_SAT or CTR_METHOD_PSEUDO_DD");
+  if (method != CTR_METHOD_DIRECT && method != CTR_METHOD_SAT &&
+      method != CTR_METHOD_PSEUDO_DD && method != CTR_METHOD_ZLINE)
+    return fail(CTR_ERR_ARG, "method must be CTR_METHOD_DIRECT, CTR_METHOD_SAT, "
+                             "CTR_METHOD_PSEUDO_DD or CTR_METHOD_ZLINE");
   P->method = method;
   return CTR_OK;
 }
@@ -970,14 +990,13 @@ int ctr_dd3_back(float x0, float y0, float z0, int DNU, int DNV, const float* xd
   const Model m = bp_model(prjMode & ~16);
   if (matched && m != Model::Branchless)
     return fail(CTR_ERR_UNSUPPORTED, "the matched adjoint exists for the branchless model only");
-  if (m == Model::ZLine)
-    return fail(CTR_ERR_UNSUPPORTED, "prjMode 3 (z-line back projector) is not implemented");
   ctr_geometry g = make_geometry(x0, y0, z0, DNU, DNV, xds, yds, zds, imgXCenter, imgYCenter,
                                  imgZCenter, hangs, hzPos, PN, XN, YN, ZN, dx, dz, mask);
   PlanGuard pg;
   int rc;
   if ((rc = ctr_plan_create(&g, gpunum, &pg.p))) return rc;
   if (m == Model::PseudoDD && (rc = ctr_plan_set_method(pg.p, CTR_METHOD_PSEUDO_DD))) return rc;
+  if (m == Model::ZLine && (rc = ctr_plan_set_method(pg.p, CTR_METHOD_ZLINE))) return rc;
   const size_t NV = (size_t)XN * YN * ZN, NS = (size_t)DNU * DNV * PN;
   DBuf dv, dp;
   if ((rc = dv.alloc(NV * sizeof(float)))) return rc;

@@ -157,10 +157,11 @@ class Plan:
 
     def set_method(self, method):
         """'direct' (footprint integration, default), 'sat' (compensated integral images)
-        -- both the branchless distance-driven model -- or 'pseudo_dd' (the reference's
-        pseudo distance-driven pair, prjMode 1)."""
+        -- both the branchless distance-driven model --, 'pseudo_dd' (the reference's
+        pseudo distance-driven pair, prjMode 1) or 'zline' (prjMode 3: pseudo-DD forward,
+        z-line back)."""
         code = {"direct": _lib.CTR_METHOD_DIRECT, "sat": _lib.CTR_METHOD_SAT,
-                "pseudo_dd": _lib.CTR_METHOD_PSEUDO_DD}[method]
+                "pseudo_dd": _lib.CTR_METHOD_PSEUDO_DD, "zline": _lib.CTR_METHOD_ZLINE}[method]
         check(lib().ctr_plan_set_method(self.handle, code))
         self.method = method
 

@@ -51,6 +51,11 @@ extern "C" {
  * view.  A different operator from the two above; FP32 operands only, reference back
  * projector only (no matched adjoint), not available to the OS-SART / CG drivers. */
 #define CTR_METHOD_PSEUDO_DD 2
+/* The reference's prjMode 3 pair: the pseudo-DD forward projector and the z-line
+ * back projector (iterativeRecon/DD3_GPU.cu:1545-1647), whose reference comment notes
+ * that it needs ZN divisible by 16: it writes the slices of the whole 16-slice blocks
+ * and leaves the rest 0, which is reproduced.  Same restrictions as PSEUDO_DD. */
+#define CTR_METHOD_ZLINE 3
 
 /* Storage of projector operands (build addition; the reference is FP32 throughout).
  * BF16 arrays are unsigned short bit patterns: the upper half of the FP32 value,
@@ -117,8 +122,8 @@ typedef struct ctr_plan ctr_plan;
 int ctr_plan_create(const ctr_geometry* geom, int device, ctr_plan** out);
 int ctr_plan_destroy(ctr_plan* plan);
 
-/* Select the projector method (CTR_METHOD_DIRECT, the default, CTR_METHOD_SAT or
- * CTR_METHOD_PSEUDO_DD) used by every later call on the plan.  The reference selects
+/* Select the projector method (CTR_METHOD_DIRECT, the default, CTR_METHOD_SAT,
+ * CTR_METHOD_PSEUDO_DD or CTR_METHOD_ZLINE) used by every later call on the plan.  The reference selects
  * the model per call with prjMode and always builds integral images for mode 0
  * (iterativeRecon/DD3_GPU.cu:1036, 1710). */
 int ctr_plan_set_method(ctr_plan* plan, int method);

@@ -1,7 +1,7 @@
 // DD3 ORACLE -- TEST INFRASTRUCTURE ONLY.
 //
-// A CPU restatement of the reference's mode-0 distance-driven projector pair and
-// of the OS-SART / CG loops that drive it.  Only the tests, the smoke check and
+// A CPU restatement of the reference's projector models (the mode-0 distance-driven
+// pair and the other prjMode values) and of the OS-SART / CG loops that drive mode 0.  Only the tests, the smoke check and
 // the CPU-baseline leg of the benchmark load this file's library; the product
 // (ctrecon_amd) never links, imports or calls it.
 //
@@ -20,6 +20,7 @@
 //   forward, pseudo-DD     iterativeRecon/DD3_GPU.cu:673-834 (prjMode 1 and 3)
 //   forward, FP64 SAT      iterativeRecon/DD3_GPU.cu:255-660 (prjMode 2)
 //   back, pseudo-DD        iterativeRecon/DD3_GPU.cu:1478-1528 (prjMode 1 and 2)
+//   back, z-line           iterativeRecon/DD3_GPU.cu:1545-1647 (prjMode 3)
 //   OS-SART                iterativeRecon/os_sart.m:13-44
 //   CG                     iterativeRecon/conjgrad.m:4-27
 //
@@ -419,8 +420,10 @@ BackConsts<T> back_consts(const Geo<T>& g) {
 }
 
 // ---------------- back, SAT transcription (DD3_GPU.cu:1397-1475) ----------------
-template <typename T, typename TO>
-void bp_sat(const Geo<T>& g, const float* prj, TO* vol, int nthreads) {
+// genSAT_of_Projection (DD3_GPU.cu:1241-1323): per view, the sinogram with a zero row
+// and column in front, integrated along v then u: sat[p][u'][v'] (W = DNV+1, H = DNU+1).
+template <typename T>
+std::vector<T> sino_sat(const Geo<T>& g, const float* prj) {
   const int DNU = g.DNU, DNV = g.DNV, W = DNV + 1, H = DNU + 1;
   std::vector<T> sat((size_t)g.PN * H * W, T(0));
   for (int p = 0; p < g.PN; ++p)
@@ -434,6 +437,13 @@ void bp_sat(const Geo<T>& g, const float* prj, TO* vol, int nthreads) {
     for (int h = 1; h < H; ++h)
       for (int w = 0; w < W; ++w) pl[(size_t)h * W + w] += pl[(size_t)(h - 1) * W + w];
   }
+  return sat;
+}
+
+template <typename T, typename TO>
+void bp_sat(const Geo<T>& g, const float* prj, TO* vol, int nthreads) {
+  const int DNU = g.DNU, DNV = g.DNV, W = DNV + 1, H = DNU + 1;
+  const std::vector<T> sat = sino_sat(g, prj);
   const BackConsts<T> k = back_consts(g);
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
@@ -603,6 +613,62 @@ void bp_pdd(const Geo<T>& g, const float* prj, T* vol) {
                   cosVal;
         }
         col[z] = summ;
+      }
+    }
+}
+
+// ---------------- z-line back (prjMode 3; DD3_GPU.cu:1545-1647, host :1723-1737) --------
+// Per voxel column, in blocks of 16 slices (only ZN / 16 whole blocks; the slices of a
+// last partial block stay 0): for every view, 17 terms at the lower faces of the slices
+// lp*16 + j, j = 0..16, each the sinogram integral over the fan-angle range [tL, tR] of
+// the column's y- (or x-) extent and the rows below the face, times the signed cell
+// factor dx |d_xy| / d_x (or / d_y) and hypot(d_z, |d_xy|) / |d_xy| evaluated at the slice
+// centre; a slice gets the difference of the sums at its upper and lower faces.
+template <typename T>
+void bp_zline(const Geo<T>& g, const float* prj, T* vol) {
+  const int DNU = g.DNU, DNV = g.DNV, W = DNV + 1, H = DNU + 1;
+  const std::vector<T> sat = sino_sat(g, prj);
+  const BackConsts<T> k = back_consts(g);
+  const int LPs = g.ZN >> 4;
+#pragma omp parallel for collapse(2) schedule(dynamic, 4)
+  for (int y = 0; y < g.YN; ++y)
+    for (int x = 0; x < g.XN; ++x) {
+      T* col = vol + ((size_t)y * g.XN + x) * g.ZN;
+      for (int z = 0; z < g.ZN; ++z) col[z] = T(0);
+      if (g.mask[y * g.XN + x] != 1) continue;
+      const T Px = (T(x) - g.ocx) * g.dx, Py = (T(y) - g.ocy) * g.dx;
+      for (int lp = 0; lp < LPs; ++lp) {
+        T summ[17] = {};
+        const T minObj = (T(lp * 16) - g.ocz) * g.dz;
+        for (int p = 0; p < g.PN; ++p) {
+          const T c = g.cs[p], s = g.sn[p];
+          const T sx = g.x0 * c - g.y0 * s, sy = g.x0 * s + g.y0 * c, sz = g.z0 + g.zsh[p];
+          const T dX = Px - sx, dY = Py - sy;
+          const T s2v = std::hypot(dX, dY);
+          T lx, ly, rx, ry, cosVal;
+          if (std::fabs(c) <= std::fabs(s)) {
+            lx = dX; ly = dY - T(0.5) * g.dx;
+            rx = dX; ry = dY + T(0.5) * g.dx;
+            cosVal = g.dx * s2v / dX;
+          } else {
+            lx = dX + T(0.5) * g.dx; ly = dY;
+            rx = dX - T(0.5) * g.dx; ry = dY;
+            cosVal = g.dx * s2v / dY;
+          }
+          const T nl = std::sqrt(lx * lx + ly * ly), nr = std::sqrt(rx * rx + ry * ry);
+          const T tL = std::asin((sy * lx - sx * ly) / nl * k.inv_sid) * k.ib + k.cU;
+          const T tR = std::asin((sy * rx - sx * ry) / nr * k.inv_sid) * k.ib + k.cU;
+          const T curang = k.S / s2v;
+          const T* pl = sat.data() + (size_t)p * H * W;
+          for (int j = 0; j <= 16; ++j) {
+            const T dirZ = minObj + T(j) * g.dz - sz;
+            const T w = std::hypot(dirZ, s2v) / s2v;
+            const T tD = (dirZ - T(0.5) * g.dz) * curang + k.cV;
+            summ[j] += (tex2(pl, W, H, tD + 1, tR + 1) - tex2(pl, W, H, tD + 1, tL + 1)) *
+                       cosVal * w;
+          }
+        }
+        for (int j = 0; j < 16; ++j) col[lp * 16 + j] = summ[j + 1] - summ[j];
       }
     }
 }
@@ -819,6 +885,11 @@ void orc_fp_pdd_f64(ORC_GEO_ARGS, const float* vol, double* prj, float dx, float
 void orc_bp_pdd_f64(ORC_GEO_ARGS, double* vol, const float* prj, float dx, float dz,
                     const uint8_t* mask) {
   bp_pdd(make_geo<double>(ORC_GEO_PASS, dx, dz, mask), prj, vol);
+}
+
+void orc_bp_zline_f64(ORC_GEO_ARGS, double* vol, const float* prj, float dx, float dz,
+                      const uint8_t* mask) {
+  bp_zline(make_geo<double>(ORC_GEO_PASS, dx, dz, mask), prj, vol);
 }
 
 void orc_fp_dp_f64(ORC_GEO_ARGS, const float* vol, double* prj, float dx, float dz,

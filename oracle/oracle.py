@@ -84,15 +84,16 @@ def fp64(g, vol64):
 def bp(g, prj, method="sat"):
     """FP64 back projection of float32 prj[p, u, v] -> float64 vol[y, x, z].
     method: 'sat' / 'direct' (reference arithmetic, DD3_GPU.cu:1397-1475) or
-    'adjoint' (exact transpose of the forward projector) or 'pdd' (pseudo distance
-    driven, DD3_GPU.cu:1478-1528)."""
+    'adjoint' (exact transpose of the forward projector), 'pdd' (pseudo distance
+    driven, DD3_GPU.cu:1478-1528) or 'zline' (z-line, DD3_GPU.cu:1545-1647)."""
     prj = np.ascontiguousarray(prj, dtype=np.float32)
     assert prj.shape == g.prj_shape
     out = np.zeros(g.vol_shape, dtype=np.float64)
     args, keep = _geo_args(g)
     m = _mask(g)
     fn = {"sat": lib().orc_bp_sat_f64, "direct": lib().orc_bp_direct_f64,
-          "adjoint": lib().orc_bp_adjoint_f64, "pdd": lib().orc_bp_pdd_f64}[method]
+          "adjoint": lib().orc_bp_adjoint_f64, "pdd": lib().orc_bp_pdd_f64,
+          "zline": lib().orc_bp_zline_f64}[method]
     fn(*args, _ptr(out), _ptr(prj), _f(g.dx), _f(g.dz), _ptr(m))
     return out
 

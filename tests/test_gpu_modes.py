@@ -1,7 +1,8 @@
 """GPU parity of the reference's other projector modes against the FP64 oracle: the
-pseudo distance-driven pair (prjMode 1 / 3 forward, 1 / 2 back) and the FP64 branchless
-forward projector (prjMode 2), through the plan API and the DD3Proj_gpu / DD3Back_gpu
-entry points.  Bar: 1e-5 relative L2 (FP32 arithmetic against FP64).
+pseudo distance-driven pair (prjMode 1 / 3 forward, 1 / 2 back), the FP64 branchless
+forward projector (prjMode 2) and the z-line back projector (prjMode 3), through the
+plan API and the DD3Proj_gpu / DD3Back_gpu entry points.  Bar: 1e-5 relative L2 (FP32
+arithmetic against FP64).
 """
 
 import numpy as np
@@ -95,6 +96,14 @@ def test_reference_entry_points_select_modes(name, oracle):
     b_pdd = oracle.bp(g, y, "pdd")
     for mode in (1, 2):
         assert rel_l2(_dd3_back(g, y, mode), b_pdd) <= TOL
+    got = _dd3_back(g, y, 3)
+    ref = oracle.bp(g, y, "zline")
+    if np.any(ref != 0):
+        err = rel_l2(got, ref)
+        _say(f"  prjMode 3 back rel-L2 vs FP64 oracle = {err:.3e}")
+        assert err <= TOL
+    else:
+        assert np.all(got == 0)
     # values outside 1..3 are mode 0
     assert rel_l2(_dd3_proj(g, vol, 7), oracle.fp(g, vol, "sat")) <= TOL
 
@@ -154,3 +163,49 @@ def test_pdd_plan_refusals(gpu_plan_factory):
         plan.conjgrad(d_x, d_b, 1)
     plan.set_method("direct")
     plan.os_sart(d_x, d_b, 2, 1)
+
+
+def _tall(ZN):
+    from ctrecon_amd import geometry as G
+
+    cfg = G.helical_config(sid=300, sdd=520, DNU=40, DNV=8, col_width=2.0, col_height=1.6,
+                           views_per_turn=36, turns=2, feed_mm_per_turn=8.0, N_xy=24, N_z=ZN,
+                           voxel_xy=2.0, voxel_z=1.0)
+    m = np.ones((24, 24), np.uint8)
+    m[np.random.default_rng(2).uniform(size=m.shape) < 0.1] = 0
+    return cfg, G.dd3_geometry(cfg, mask=m)
+
+
+@pytest.mark.parametrize("ZN", [37, 64, 150])
+def test_zline_parity(ZN, oracle, gpu_plan_factory):
+    """ZN 37 (two whole 16-slice blocks, 5 slices left 0), 64 (one z group), 150 (three
+    z groups in one wave, 6 slices left 0)."""
+    _say(f"[z-line] ZN={ZN}")
+    cfg, g = _tall(ZN)
+    plan = gpu_plan_factory(g)
+    plan.set_method("zline")
+    y = sinogram(g)
+    got = plan.back(y, kind="ref")
+    ref = oracle.bp(g, y, "zline")
+    err = rel_l2(got, ref)
+    _say(f"  z-line rel-L2 vs FP64 oracle = {err:.3e}")
+    assert err <= TOL
+    NZ = 16 * (ZN // 16)
+    assert np.all(got[:, :, NZ:] == 0)
+    m = g.mask.reshape(g.YN, g.XN)
+    assert np.all(got[m != 1] == 0)
+    # the prjMode 3 forward projector is the pseudo-DD one
+    vol = volume(g)
+    assert rel_l2(plan.forward(vol), oracle.fp(g, vol, "pdd")) <= TOL
+
+
+def test_zline_clinical_view(oracle, gpu_plan_factory):
+    """Clinical geometry (512^2 x 256, 888 x 64): z-line back projection of one view."""
+    cfg, g = clinical()
+    plan = gpu_plan_factory(g)
+    plan.set_method("zline")
+    y = sinogram(g.subset([0]))
+    got = plan.back(y, [0], kind="ref")
+    err = rel_l2(got, oracle.bp(g.subset([0]), y, "zline"))
+    _say(f"[z-line clinical view 0] rel-L2 = {err:.3e}")
+    assert err <= TOL

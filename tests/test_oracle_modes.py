@@ -244,3 +244,84 @@ def test_fp_dp_is_half_cell_shifted_mode0(case, xmarch, oracle):
     assert rel_l2(got, ref) < (1e-10 if case is tiny else 1e-6)
     # and it is not the unshifted model
     assert rel_l2(got, oracle.fp(g, vol, "direct")) > 1e-3
+
+
+# ---------------------------------------------------------------- z-line back (prjMode 3)
+def _zline_numpy(g, prj):
+    """Sum over views of cosVal w(j) G(tD_j) at the lower faces j of the slices, G the
+    exact integral of the piecewise-constant view over the fan-angle range of the
+    column's extent and the rows below tD; slice k gets S(k+1) - S(k) for the whole
+    16-slice blocks, 0 above them."""
+    ocx, ocy, ocz = _oc(g)
+    x0, y0, z0 = g.x0, g.y0, g.z0
+    bx, by, bz = _bounds(g.xds), _bounds(g.yds), _bounds(g.zds)
+
+    def ang(ax, ay, bx_, by_):
+        return math.atan2(ax * by_ - ay * bx_, ax * bx_ + ay * by_)
+
+    beta0 = ang(-x0, -y0, bx[0] - x0, by[0] - y0)
+    dbeta = ang(bx[0] - x0, by[0] - y0, bx[-1] - x0, by[-1] - y0) / g.DNU
+    ddv = (bz[-1] - bz[0]) / g.DNV
+    S2D = math.hypot(float(g.xds[0]) - x0, float(g.yds[0]) - y0)
+    NZ = 16 * (g.ZN // 16)
+    zf = (np.arange(NZ + 1) - ocz) * g.dz  # slice centres 0..NZ
+    cum = np.concatenate([np.zeros(prj.shape[:2] + (1,)), np.cumsum(prj, axis=2)], axis=2)
+    grid = np.arange(g.DNV + 1)
+    m = g.mask.reshape(g.YN, g.XN)
+    out = np.zeros(g.vol_shape)
+    for y in range(g.YN):
+        for x in range(g.XN):
+            if m[y, x] != 1 or NZ == 0:
+                continue
+            Px, Py = (x - ocx) * g.dx, (y - ocy) * g.dx
+            S = np.zeros(NZ + 1)
+            for p in range(g.PN):
+                c, s, Sv, *_ = _view(g, p)
+                dX, dY = Px - Sv[0], Py - Sv[1]
+                s2v = math.hypot(dX, dY)
+                if abs(c) <= abs(s):
+                    L, R, cosv = (dX, dY - g.dx / 2), (dX, dY + g.dx / 2), g.dx * s2v / dX
+                else:
+                    L, R, cosv = (dX + g.dx / 2, dY), (dX - g.dx / 2, dY), g.dx * s2v / dY
+                tL = (ang(-Sv[0], -Sv[1], *L) - beta0) / dbeta - 0.5
+                tR = (ang(-Sv[0], -Sv[1], *R) - beta0) / dbeta - 0.5
+                lo = min(max(min(tL, tR), -0.5), g.DNU - 0.5)
+                hi = min(max(max(tL, tR), -0.5), g.DNU - 0.5)
+                sgn = 1.0 if tR >= tL else -1.0
+                dirZ = zf - Sv[2]
+                w = np.hypot(dirZ, s2v) / s2v
+                tD = (z0 + (dirZ - g.dz / 2) * S2D / s2v - bz[0]) / ddv - 0.5
+                G = np.zeros(NZ + 1)
+                for u in range(g.DNU):
+                    wu = min(hi, u + 0.5) - max(lo, u - 0.5)
+                    if wu > 0:
+                        G += wu * np.interp(tD + 0.5, grid, cum[p, u])
+                S += sgn * cosv * w * G
+            out[y, x, :NZ] = S[1:] - S[:-1]
+    return out
+
+
+def _tall_tiny():
+    cfg = G.helical_config(sid=60, sdd=100, DNU=12, DNV=4, col_width=4.0, col_height=3.0,
+                           views_per_turn=8, turns=2, feed_mm_per_turn=6.0, N_xy=8, N_z=32,
+                           voxel_xy=4.0, voxel_z=1.5)
+    return cfg, G.dd3_geometry(cfg)
+
+
+@pytest.mark.parametrize("case", [_tall_tiny, ragged])
+def test_zline_vs_numpy(case, oracle):
+    """ragged has ZN = 37: two whole 16-slice blocks, the last 5 slices stay 0."""
+    cfg, g = case()
+    y = sinogram(g)
+    got = oracle.bp(g, y, "zline")
+    ref = _zline_numpy(g, y.astype(np.float64))
+    assert rel_l2(got, ref) < 1e-10
+    NZ = 16 * (g.ZN // 16)
+    assert np.all(got[:, :, NZ:] == 0)
+    m = g.mask.reshape(g.YN, g.XN)
+    assert np.all(got[m != 1] == 0) and np.any(got[m == 1] != 0)
+
+
+def test_zline_fewer_than_16_slices_is_zero(oracle):
+    cfg, g = tiny()  # ZN = 8: no whole block
+    assert np.all(oracle.bp(g, sinogram(g), "zline") == 0)
