@@ -158,6 +158,69 @@ def batched_workload(args):
     print(json.dumps(out))
 
 
+def projectors_workload(args):
+    """Every projector model of the library on the clinical geometry (512x512x256, 888x64,
+    3936 views): one OS-SART subset (every 20th view, 197 views) forward- and back-projected
+    per launch.  Times each launch (mean of K after one warm-up, synchronised wall clock
+    around the launches) and prices it with the byte model B_view = 4 N_cone + 4 DNU DNV
+    (N_cone: masked voxels inside the view's cone)."""
+    from ctrecon_amd import geometry as G
+    from ctrecon_amd.plan import DeviceArray, Plan, to_bf16
+
+    g = G.dd3_geometry(G.config_clinical(), mask=G.fov_mask(512))
+    plan = Plan(g, 0)
+    rng = np.random.default_rng(0)
+    vs = args.subsets
+    vc = len(range(0, g.PN, vs))
+    dv = DeviceArray.from_host(rng.uniform(0, 1, g.vol_shape).astype(np.float32))
+    dp = DeviceArray.from_host(rng.uniform(0, 1, (vc, g.DNU, g.DNV)).astype(np.float32))
+    dvo = DeviceArray(g.vol_shape)
+    dv16, dp16 = to_bf16(dv), to_bf16(dp)
+    cone = float(np.sum(plan.cone_voxels(0, vs, vc)))
+    bytes_launch = 4.0 * cone + 4.0 * g.DNU * g.DNV * vc
+    K = max(args.steps, 3)
+    legs = [
+        ("fp_branchless_direct", "direct", lambda: plan.fp(dv, dp, 0, vs, vc)),
+        ("bp_branchless_reference", "direct", lambda: plan.bp(dp, dvo, 0, vs, vc, "ref")),
+        ("bp_matched_adjoint", "direct", lambda: plan.bp(dp, dvo, 0, vs, vc, "matched")),
+        ("fp_branchless_bf16", "direct", lambda: plan.fp_bf16(dv16, dp16, 0, vs, vc)),
+        ("bp_branchless_bf16", "direct", lambda: plan.bp_bf16(dp16, dvo, 0, vs, vc, "ref")),
+        ("fp_branchless_sat", "sat", lambda: plan.fp(dv, dp, 0, vs, vc)),
+        ("bp_branchless_sat", "sat", lambda: plan.bp(dp, dvo, 0, vs, vc, "ref")),
+        ("fp_pseudo_dd", "pseudo_dd", lambda: plan.fp(dv, dp, 0, vs, vc)),
+        ("bp_pseudo_dd", "pseudo_dd", lambda: plan.bp(dp, dvo, 0, vs, vc, "ref")),
+        ("bp_zline", "zline", lambda: plan.bp(dp, dvo, 0, vs, vc, "ref")),
+    ]
+    table = {}
+    for name, method, fn in legs:
+        plan.set_method(method)
+        fn()
+        plan.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(K):
+            fn()
+        plan.synchronize()
+        ms = (time.perf_counter() - t0) / K * 1e3
+        table[name] = {"ms": round(ms, 3), "projections_per_s": round(vc / (ms / 1e3), 1),
+                       "hbm_frac": round(bytes_launch / (ms / 1e3) / HBM_PEAK, 4)}
+        log(f"{name}: {ms:.2f} ms per {vc}-view launch")
+    out = {
+        "metric": "projections per second per projector model, 512x512x256 helical, "
+                  "888x64 detector",
+        "value": table["fp_branchless_direct"]["projections_per_s"],
+        "unit": "projections/s",
+        "n_gpus": 1,
+        "steps": K,
+        "higher_is_better": True,
+        "dtype": "f32",
+        "data": "synthetic: uniform random volume and sinogram",
+        "config": {"workload": f"one {vc}-view subset (every {vs}th view of 3936) per launch"},
+        "bytes_per_launch_model": bytes_launch,
+        "kernels": table,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,8 +235,10 @@ def main():
     ap.add_argument("--form", default="both", choices=["both", "plain", "learned"])
     ap.add_argument("--weights", default=os.path.join(ROOT, "ctrecon_amd", "weights",
                                                       "warm_start.npz"))
-    ap.add_argument("--workload", default="clinical", choices=["clinical", "batched"],
-                    help="clinical: the headline solve; batched: 16 uneven scans, one sweep")
+    ap.add_argument("--workload", default="clinical",
+                    choices=["clinical", "batched", "projectors"],
+                    help="clinical: the headline solve; batched: 16 uneven scans, one sweep; "
+                         "projectors: every projector model on one clinical subset")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run the plain copy kernel (byte-counter calibration)")
     args = ap.parse_args()
@@ -204,6 +269,9 @@ def main():
 
     if args.workload == "batched":
         batched_workload(args)
+        return
+    if args.workload == "projectors":
+        projectors_workload(args)
         return
 
     cfg = G.config_clinical()
