@@ -389,7 +389,8 @@ def main():
         bytes_per_view = 4.0 * cone + 4.0 * frame
         per_launch_bytes = float(bytes_per_view.sum()) / args.subsets
         views_per_launch = g.PN / args.subsets
-        dominant = "bp_ref_kernel" if args.bp == "ref" else "bp_matched_kernel"
+        # the reference back projector runs windowed for columns up to 2048 voxels
+        dominant = "bp_win_kernel" if args.bp == "ref" else "bp_matched_kernel"
         achieved = per_launch_bytes / (bp_avg / 1e3)
         traffic = None
         tpath = os.path.join(ROOT, "profiles", "traffic.json")

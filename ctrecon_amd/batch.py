@@ -41,7 +41,8 @@ class Batch:
 
     def set_tuning(self, fp_colw: bool, fp_grid: int, bp_zg: int, bp_grid: int,
                    fp_strided=False, bp_strided=False):
-        """Launch shapes; a grid of 0 runs one workgroup per work item."""
+        """Launch shapes; a grid of 0 runs one workgroup per work item; bp_zg: z groups
+        per lane (1..4), 0 each scan's own, 5 the windowed back projector."""
         check(lib().ctr_batch_set_tuning(self.handle, int(bool(fp_colw)), fp_grid,
                                          int(bool(fp_strided)), bp_zg, bp_grid,
                                          int(bool(bp_strided))))

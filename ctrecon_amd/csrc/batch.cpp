@@ -62,13 +62,16 @@ struct ctr_batch {
   Mem sums, zeros;  // sums: [n][osNum][2]; zeros: the largest volume (tuning)
   Mem fpItems, redItems;
   std::vector<int> fpTotal;
-  // back-projection work lists by z groups per lane (class c = 1..4): the scans of the
-  // class, and per subset step their items (and timing copies with a zero column sum)
+  // back-projection work lists by class: c = 1..4 z groups per lane, c = 0 the windowed
+  // form (one wave per column over all z); the scans of the class, and per subset step
+  // their items (and timing copies with a zero column sum)
   std::vector<int> clsScans[5];
   Mem bpItems[5], tuneBpItems[5];
   std::vector<int> bpTotal[5];
+  int winZ = 0;  // tallest column of the windowed class
   bool colw = false;
-  // zg: z groups per lane for every scan, or 0 for each scan's own ceil(ZN / 64)
+  // zg: z groups per lane for every scan (1..4), 0 for each scan's own ceil(ZN / 64), or
+  // kZgWindow for the windowed form (scans with columns taller than kWinMaxZ: their own)
   int fpGrid = 0, zg = 0, bpGrid = 0;  // grids: persistent workgroups, <= 0 one per item
   int fpStrided = 0, bpStrided = 0;
   hipStream_t st = nullptr;
@@ -85,8 +88,12 @@ struct ctr_batch {
 
 namespace {
 
+constexpr int kZgWindow = 5;
+
+// back-projection class of a scan (0: windowed)
 int scan_zg(const ctr_batch& B, const Plan& P) {
-  return B.zg > 0 ? B.zg : std::min(4, (P.d.ZN + 63) / 64);
+  if (B.zg == kZgWindow && P.d.ZN <= kWinMaxZ) return 0;
+  return (B.zg > 0 && B.zg < kZgWindow) ? B.zg : std::min(4, (P.d.ZN + 63) / 64);
 }
 
 // Descriptor tables for the current tuning (zg and colw change the work lists).
@@ -96,11 +103,16 @@ int build_tables(ctr_batch& B) {
   std::vector<FpBatchItem<T>> fp((size_t)S * n);
   std::vector<ReduceItem> red((size_t)S * n);
   B.fpTotal.assign(S, 0);
-  for (int c = 1; c <= 4; ++c) {
+  for (int c = 0; c <= 4; ++c) {
     B.clsScans[c].clear();
     B.bpTotal[c].assign(S, 0);
   }
-  for (int i = 0; i < n; ++i) B.clsScans[scan_zg(B, *B.scans[i]->P)].push_back(i);
+  B.winZ = 0;
+  for (int i = 0; i < n; ++i) {
+    const int c = scan_zg(B, *B.scans[i]->P);
+    B.clsScans[c].push_back(i);
+    if (c == 0) B.winZ = std::max(B.winZ, B.scans[i]->P->d.ZN);
+  }
   auto up = [&](Mem& m, const void* src, size_t bytes) -> int {
     if (m.p) {
       (void)hipFree(m.p);
@@ -130,7 +142,7 @@ int build_tables(ctr_batch& B) {
     B.fpTotal[s] = fb;
   }
   int rc;
-  for (int cz = 1; cz <= 4; ++cz) {
+  for (int cz = 0; cz <= 4; ++cz) {
     const std::vector<int>& L = B.clsScans[cz];
     if (L.empty()) continue;
     const size_t m = L.size();
@@ -146,7 +158,7 @@ int build_tables(ctr_batch& B) {
                          B.bf ? c.xs.h() : nullptr);
         b.begin = bb;
         b.colBlocks = (P.d.XN * P.d.YN + 3) / 4;
-        const int zblocks = (P.d.ZN + 64 * cz - 1) / (64 * cz);
+        const int zblocks = cz == 0 ? 1 : (P.d.ZN + 64 * cz - 1) / (64 * cz);
         bb += vc > 0 ? b.colBlocks * zblocks : 0;
         BpBatchItem<T>& t = tbp[(size_t)s * m + j];
         t = b;
@@ -171,12 +183,15 @@ int step_fp(ctr_batch& B, int s) {
 
 template <typename T>
 int step_bp(ctr_batch& B, int s, bool tuning) {
-  for (int cz = 1; cz <= 4; ++cz) {
+  for (int cz = 0; cz <= 4; ++cz) {
     const size_t m = B.clsScans[cz].size();
     if (!m) continue;
     const Mem& mem = tuning ? B.tuneBpItems[cz] : B.bpItems[cz];
-    int rc = launch_bp_batch<T>((const BpBatchItem<T>*)mem.p + (size_t)s * m, (int)m,
-                                B.bpTotal[cz][s], cz, B.bpGrid, B.bpStrided, B.st);
+    const BpBatchItem<T>* items = (const BpBatchItem<T>*)mem.p + (size_t)s * m;
+    int rc = cz == 0 ? launch_bp_batch_win<T>(items, (int)m, B.bpTotal[0][s], B.winZ, B.bpGrid,
+                                              B.bpStrided, B.st)
+                     : launch_bp_batch<T>(items, (int)m, B.bpTotal[cz][s], cz, B.bpGrid,
+                                          B.bpStrided, B.st);
     if (rc) return rc;
   }
   return CTR_OK;
@@ -257,7 +272,7 @@ int autotune(ctr_batch& B) {
   int bzg = 0;
   bgrid = 0;
   bstr = 0;
-  const int zgs[4] = {0, 1, 2, 4};  // 0: each scan's own ceil(ZN / 64)
+  const int zgs[5] = {kZgWindow, 0, 1, 2, 4};  // 0: each scan's own ceil(ZN / 64)
   for (int zg : zgs) {
     if (zg > 1 && 64 * (zg / 2) >= maxZN) break;  // more groups than any column needs
     B.zg = zg;
@@ -395,7 +410,7 @@ int ctr_batch_set_graph(ctr_batch* B, int enable) {
 int ctr_batch_set_tuning(ctr_batch* B, int fp_colw, int fp_grid, int fp_strided, int bp_zg,
                          int bp_grid, int bp_strided) {
   if (!B) return fail(CTR_ERR_ARG, "NULL batch");
-  if (bp_zg < 0 || bp_zg > 4 || fp_grid < 0 || bp_grid < 0)
+  if (bp_zg < 0 || bp_zg > kZgWindow || fp_grid < 0 || bp_grid < 0)
     return fail(CTR_ERR_ARG, "bad tuning");
   if (fp_colw)
     for (auto& c : B->scans)
@@ -425,7 +440,7 @@ int ctr_batch_get_info(const ctr_batch* B, ctr_batch_info* info) {
   long long f = 0, b = 0;
   for (int s = 0; s < B->osNum; ++s) {
     f += B->fpTotal[s];
-    for (int cz = 1; cz <= 4; ++cz)
+    for (int cz = 0; cz <= 4; ++cz)
       if (!B->bpTotal[cz].empty()) b += B->bpTotal[cz][s];
   }
   info->fp_items = f;

@@ -11,6 +11,10 @@
 
 namespace ctr {
 
+// Largest column height (voxels) the windowed back projector keeps in LDS:
+// 4 waves x kWinMaxZ floats = 32 KiB per workgroup.
+constexpr int kWinMaxZ = 2048;
+
 // Per-view constants (computed on the host in FP64, stored FP32).
 //   (sx, sy, sz)  rotated source position (DD3_GPU.cu:875-878, 1441-1444)
 //   az            (z0 + zshift)/dz + ocz: z index of the source plane offset
@@ -198,6 +202,10 @@ int launch_fp_batch(const FpBatchItem<T>* items, int nscan, int total, bool colw
 template <typename T>
 int launch_bp_batch(const BpBatchItem<T>* items, int nscan, int total, int zg, int grid,
                     int strided, hipStream_t st);
+// windowed form: one work item per block of 4 columns over all z; zmax = tallest column
+template <typename T>
+int launch_bp_batch_win(const BpBatchItem<T>* items, int nscan, int total, int zmax, int grid,
+                        int strided, hipStream_t st);
 int launch_reduce_batch(const ReduceItem* items, int nscan, hipStream_t st);
 int launch_to_bf16(uint16_t* dst, const float* src, size_t n, hipStream_t st);
 // the solvers' dithered BF16 shadow of an iterate (to_bf16_dither)

@@ -283,10 +283,124 @@ __global__ __launch_bounds__(kBlock) void fp_batch_kernel(const FpBatchItem<T>* 
 // the updated value rounded to BF16; otherwise vol = acc.
 // T: storage type of prj (float, or uint16_t for BF16); vol is FP32.
 // ---------------------------------------------------------------------------
+// Per-(column, view) constants of the reference back projector, uniform over a wave:
+// the fan-angle shadow (FP64) as first cell u0, cell count nu and first / last column
+// weights, the cell-width factor with the sign of the corner difference folded in, and
+// the row-shadow constants (rows of a voxel at dZ = Pz - sz: [dZ rlS + cA, dZ rlS + cB]).
+struct BpView {
+  float rlS, cA, cB, sz, l2, cws, wu0, fh;
+  int u0, nu;
+  bool uvis;
+};
+
+template <typename T>
+__device__ __forceinline__ BpView bp_view(const BpArgs<T>& A, int p, double Px, double Py,
+                                          float Pxf, float Pyf) {
+  const Dims& d = A.d;
+  const BpConsts& c = A.c;
+  const BpConsts64& c64 = A.c64;
+  BpView r{};
+  const ViewP vp = gload(A.views, p);
+  // row shadow, FP32 (DD3_GPU.cu:1446-1450): t = (dZ +- hz) * S / l + cV
+  const float dXf = Pxf - vp.sx, dYf = Pyf - vp.sy;
+  const float l2f = dXf * dXf + dYf * dYf;
+  const float rlS = __builtin_amdgcn_rsqf(l2f) * c.S;
+  const bool rneg = rlS < 0.f;  // rows then run backwards: swap the bounds
+  // fan-angle shadow, FP64 (DD3_GPU.cu:1447-1464); sy*dX - sx*dY == sy*Px - sx*Py
+  const double2 s64 = gload(A.src64, p);
+  const double dX = Px - s64.x, dY = Py - s64.y;
+  const double l2 = dX * dX + dY * dY;
+  const double rl = rsqrt(l2);
+  const double alpha = asin_fan((s64.y * Px - s64.x * Py) * c64.inv_sid * rl);
+  const double w = vp.wx ? dX : dY;
+  const double dA = w * (rl * rl) * (c64.dx * 0.5);
+  const double tL = (alpha - dA) * c64.ib + c64.cU;
+  const double tR = (alpha + dA) * c64.ib + c64.cU;
+  const double ulo = fmax(fmin(tL, tR), -0.5);
+  const double uhi = fmin(fmax(tL, tR), (double)d.DNU - 0.5);
+  r.uvis = ulo < uhi;
+  if (r.uvis) {
+    const int u0 = (int)floor(ulo + 0.5);
+    const int nu = min((int)floor(uhi + 0.5), d.DNU - 1) - u0 + 1;
+    // column weights: first / last partial, interior 1 (fractions taken in FP64)
+    const float fl = (float)(((double)u0 + 0.5) - ulo);
+    r.fh = (float)(uhi - ((double)(u0 + nu - 1) - 0.5));
+    r.wu0 = (nu == 1) ? (float)(uhi - ulo) : fl;
+    // sign of the reference's corner difference, (tR - tL) and (tU - tD), folded
+    // into the cell-width factor (a sign flip is exact)
+    const float sgn = ((tR < tL) != rneg) ? -1.f : 1.f;
+    r.cws = sgn * (c.dx * __builtin_amdgcn_rcpf((float)w));
+    r.u0 = u0;
+    r.nu = nu;
+  }
+  const float cUp = fmaf(c.hz, rlS, c.cV), cDp = fmaf(-c.hz, rlS, c.cV);
+  r.cA = rneg ? cUp : cDp;  // lower row bound
+  r.cB = rneg ? cDp : cUp;  // upper row bound
+  r.rlS = rlS;
+  r.sz = vp.sz;
+  r.l2 = l2f;
+  return r;
+}
+
+// The voxel's shadow [vlo, vhi] (rows) x cells u0 .. u0 + nu - 1 integrated over the
+// piecewise-constant view pp (first / last column weights wu0, fh).
+template <typename T>
+__device__ __forceinline__ float bp_cell_sum(const Buf<T>& pp, int u0, int nu, float wu0,
+                                             float fh, float vlo, float vhi, int DNV) {
+  const int v0 = (int)floorf(vlo + 0.5f);
+  const int nv = min((int)floorf(vhi + 0.5f), DNV - 1) - v0 + 1;
+  float wv0, wvl;
+  edge_weights(vlo, vhi, v0, nv, wv0, wvl);
+  float s = 0.f;
+  if (nu <= 4 && nv <= 3) {
+    const float wv1 = (nv == 2) ? wvl : (nv == 3 ? 1.f : 0.f);
+    const float wv2 = (nv == 3) ? wvl : 0.f;
+    const int t1 = nv > 1 ? 1 : 0, t2 = nv > 2 ? 2 : 0;
+    int o = u0 * DNV + v0;
+#pragma unroll
+    for (int iu = 0; iu < 4; ++iu) {
+      if (iu < nu) {
+        const float wu = (iu == 0) ? wu0 : (iu == nu - 1 ? fh : 1.f);
+        float gsum = wv0 * pp[o];
+        gsum = fmaf(wv1, pp[o + t1], gsum);
+        gsum = fmaf(wv2, pp[o + t2], gsum);
+        s = fmaf(wu, gsum, s);
+        o += DNV;
+      }
+    }
+  } else {
+    for (int iu = 0; iu < nu; ++iu) {
+      const float wu = (iu == 0) ? wu0 : (iu == nu - 1 ? fh : 1.f);
+      const int o = (u0 + iu) * DNV;
+      float gsum = 0.f;
+      for (int iv = 0; iv < nv; ++iv) {
+        const float wv = (iv == 0) ? wv0 : (iv == nv - 1 ? wvl : 1.f);
+        gsum = fmaf(wv, pp[o + v0 + iv], gsum);
+      }
+      s = fmaf(wu, gsum, s);
+    }
+  }
+  return s;
+}
+
+// Final value of voxel z of a column: vol = acc, or the OS-SART update
+// vol += acc * invCol (and its BF16 shadow).
+template <bool UPDATE, typename T>
+__device__ __forceinline__ void bp_store(const BpArgs<T>& A, CTR_GLOBAL float* vcol, int col,
+                                         int z, float acc) {
+  if constexpr (UPDATE) {
+    const size_t i = (size_t)col * A.d.ZN + z;
+    const float nx = fmaf(acc, gp(A.invCol)[i], vcol[z]);
+    vcol[z] = nx;
+    if (A.shadow) gp(A.shadow)[i] = to_bf16_dither(nx, i);
+  } else {
+    vcol[z] = acc;
+  }
+}
+
 template <bool UPDATE, int ZG, typename T>
 __device__ __forceinline__ void bp_ref_body(const BpArgs<T>& A, int col, int zblk) {
   const Dims& d = A.d;
-  const BpConsts& c = A.c;
   const BpConsts64& c64 = A.c64;
   const int lane = threadIdx.x & 63;
   if (col >= d.XN * d.YN) return;
@@ -305,15 +419,12 @@ __device__ __forceinline__ void bp_ref_body(const BpArgs<T>& A, int col, int zbl
   const double Px = ((double)x - c64.ocx) * c64.dx;
   const double Py = ((double)y - c64.ocy) * c64.dx;
   const float Pxf = (float)Px, Pyf = (float)Py;
-  const float Pz0 = ((float)zb - d.ocz) * d.dz;        // this lane, group 0
-  const float dzg = 64.f * d.dz;                      // z step between groups
   // z extent of the wave's voxel centres (uniform), for the whole-column view test
   const float Zlo = ((float)zb0 - d.ocz) * d.dz;
   const float Zhi = ((float)min(zb0 + 64 * ZG, d.ZN) - 1.f - d.ocz) * d.dz;
   float acc[ZG];
 #pragma unroll
   for (int g = 0; g < ZG; ++g) acc[g] = 0.f;
-  const double DNUh = (double)d.DNU - 0.5;
   const float DNVh = (float)d.DNV - 0.5f;
   const int DNV = d.DNV;
   const uint32_t frame = (uint32_t)(d.DNU * d.DNV);
@@ -323,131 +434,146 @@ __device__ __forceinline__ void bp_ref_body(const BpArgs<T>& A, int col, int zbl
   // readlane into scalar registers.  The arithmetic per view is unchanged.
   for (int k0 = 0; k0 < A.vc; k0 += 64) {
     const int kl = k0 + lane;
-    float q_rlS = 0.f, q_cA = 0.f, q_cB = 0.f, q_sz = 0.f, q_l2 = 0.f, q_cws = 0.f;
-    float q_wu0 = 0.f, q_fh = 0.f;
-    int q_u0 = 0, q_nu = 0;
+    BpView q{};
     bool vis = false;
     if (kl < A.vc) {
-      const int p = A.vb + kl * A.vs;
-      const ViewP vp = gload(A.views, p);
-      // row shadow, FP32 (DD3_GPU.cu:1446-1450): t = (dZ +- hz) * S / l + cV; first the
-      // whole column's row range, to skip views it does not meet
-      const float dXf = Pxf - vp.sx, dYf = Pyf - vp.sy;
-      const float l2f = dXf * dXf + dYf * dYf;
-      const float rlS = __builtin_amdgcn_rsqf(l2f) * c.S;
-      const bool rneg = rlS < 0.f;  // rows then run backwards: swap the bounds
-      const float ta = fmaf(Zlo - vp.sz - c.hz, rlS, c.cV);
-      const float tb = fmaf(Zhi - vp.sz + c.hz, rlS, c.cV);
-      const bool zvis = fmaxf(ta, tb) > -0.5f && fminf(ta, tb) < DNVh;
-      // fan-angle shadow, FP64 (DD3_GPU.cu:1447-1464); sy*dX - sx*dY == sy*Px - sx*Py
-      const double2 s64 = gload(A.src64, p);
-      const double dX = Px - s64.x, dY = Py - s64.y;
-      const double l2 = dX * dX + dY * dY;
-      const double rl = rsqrt(l2);
-      const double alpha = asin_fan((s64.y * Px - s64.x * Py) * c64.inv_sid * rl);
-      const double w = vp.wx ? dX : dY;
-      const double dA = w * (rl * rl) * (c64.dx * 0.5);
-      const double tL = (alpha - dA) * c64.ib + c64.cU;
-      const double tR = (alpha + dA) * c64.ib + c64.cU;
-      const double ulo = fmax(fmin(tL, tR), -0.5);
-      const double uhi = fmin(fmax(tL, tR), DNUh);
-      vis = zvis && ulo < uhi;
-      if (vis) {
-        const int u0 = (int)floor(ulo + 0.5);
-        const int nu = min((int)floor(uhi + 0.5), d.DNU - 1) - u0 + 1;
-        // column weights: first / last partial, interior 1 (fractions taken in FP64)
-        const float fl = (float)(((double)u0 + 0.5) - ulo);
-        q_fh = (float)(uhi - ((double)(u0 + nu - 1) - 0.5));
-        q_wu0 = (nu == 1) ? (float)(uhi - ulo) : fl;
-        // sign of the reference's corner difference, (tR - tL) and (tU - tD), folded
-        // into the cell-width factor (a sign flip is exact)
-        const float sgn = ((tR < tL) != rneg) ? -1.f : 1.f;
-        q_cws = sgn * (c.dx * __builtin_amdgcn_rcpf((float)w));
-        const float cUp = fmaf(c.hz, rlS, c.cV), cDp = fmaf(-c.hz, rlS, c.cV);
-        q_cA = rneg ? cUp : cDp;  // lower row bound
-        q_cB = rneg ? cDp : cUp;  // upper row bound
-        q_rlS = rlS;
-        q_sz = vp.sz;
-        q_l2 = l2f;
-        q_u0 = u0;
-        q_nu = nu;
-      }
+      q = bp_view(A, A.vb + kl * A.vs, Px, Py, Pxf, Pyf);
+      // the whole group range of rows, to skip views the wave's voxels do not meet
+      const float ta = fmaf(Zlo - q.sz, q.rlS, q.cA), tb = fmaf(Zhi - q.sz, q.rlS, q.cB);
+      vis = q.uvis && fmaxf(ta, tb) > -0.5f && fminf(ta, tb) < DNVh;
     }
     uint64_t live = __ballot(vis);
     while (live) {
       const int j = __builtin_ctzll(live);
       live &= live - 1;
       const int k = k0 + j;
-      const float rlS = rdl(q_rlS, j), cA = rdl(q_cA, j), cB = rdl(q_cB, j);
-      const float vsz = rdl(q_sz, j), l2f = rdl(q_l2, j), cws = rdl(q_cws, j);
-      const float wu0 = rdl(q_wu0, j), fh = rdl(q_fh, j);
-      const int u0 = __builtin_amdgcn_readlane(q_u0, j), nu = __builtin_amdgcn_readlane(q_nu, j);
+      const float rlS = rdl(q.rlS, j), cA = rdl(q.cA, j), cB = rdl(q.cB, j);
+      const float vsz = rdl(q.sz, j), l2f = rdl(q.l2, j), cws = rdl(q.cws, j);
+      const float wu0 = rdl(q.wu0, j), fh = rdl(q.fh, j);
+      const int u0 = __builtin_amdgcn_readlane(q.u0, j), nu = __builtin_amdgcn_readlane(q.nu, j);
       const Buf<T> pp(A.prj + (size_t)k * frame, frame);
 #pragma unroll
       for (int g = 0; g < ZG; ++g) {
-      const float dZ = fmaf((float)g, dzg, Pz0) - vsz;
-      const float vlo = fmaxf(fmaf(dZ, rlS, cA), -0.5f);
-      const float vhi = fminf(fmaf(dZ, rlS, cB), DNVh);
-      const bool vok = zb + 64 * g < d.ZN && vlo < vhi;
-      if (!__any(vok)) continue;
-      if (!vok) continue;
-      const float cosVal = cws * __builtin_amdgcn_sqrtf(fmaf(dZ, dZ, l2f));
-      const int v0 = (int)floorf(vlo + 0.5f);
-      const int nv = min((int)floorf(vhi + 0.5f), DNV - 1) - v0 + 1;
-      float wv0, wvl;
-      edge_weights(vlo, vhi, v0, nv, wv0, wvl);
-      float s = 0.f;
-      if (nu <= 4 && nv <= 3) {
-        const float wv1 = (nv == 2) ? wvl : (nv == 3 ? 1.f : 0.f);
-        const float wv2 = (nv == 3) ? wvl : 0.f;
-        const int t1 = nv > 1 ? 1 : 0, t2 = nv > 2 ? 2 : 0;
-        int o = u0 * DNV + v0;
-#pragma unroll
-        for (int iu = 0; iu < 4; ++iu) {
-          if (iu < nu) {
-            const float wu = (iu == 0) ? wu0 : (iu == nu - 1 ? fh : 1.f);
-            float gsum = wv0 * pp[o];
-            gsum = fmaf(wv1, pp[o + t1], gsum);
-            gsum = fmaf(wv2, pp[o + t2], gsum);
-            s = fmaf(wu, gsum, s);
-            o += DNV;
-          }
-        }
-      } else {
-        for (int iu = 0; iu < nu; ++iu) {
-          const float wu = (iu == 0) ? wu0 : (iu == nu - 1 ? fh : 1.f);
-          const int o = (u0 + iu) * DNV;
-          float gsum = 0.f;
-          for (int iv = 0; iv < nv; ++iv) {
-            const float wv = (iv == 0) ? wv0 : (iv == nv - 1 ? wvl : 1.f);
-            gsum = fmaf(wv, pp[o + v0 + iv], gsum);
-          }
-          s = fmaf(wu, gsum, s);
-        }
-      }
-      acc[g] = fmaf(cosVal, s, acc[g]);
+        // voxel centre offset as in bp_win_body: one explicit fma, so that neither body
+        // can round it differently (a hoisted product would be rounded on its own)
+        const float dZ = fmaf((float)(zb + 64 * g) - d.ocz, d.dz, -vsz);
+        const float vlo = fmaxf(fmaf(dZ, rlS, cA), -0.5f);
+        const float vhi = fminf(fmaf(dZ, rlS, cB), DNVh);
+        const bool vok = zb + 64 * g < d.ZN && vlo < vhi;
+        if (!__any(vok)) continue;
+        if (!vok) continue;
+        const float cosVal = cws * __builtin_amdgcn_sqrtf(fmaf(dZ, dZ, l2f));
+        acc[g] = fmaf(cosVal, bp_cell_sum(pp, u0, nu, wu0, fh, vlo, vhi, DNV), acc[g]);
       }
     }
   }
 #pragma unroll
   for (int g = 0; g < ZG; ++g) {
     const int z = zb + 64 * g;
-    if (z >= d.ZN) continue;
-    if constexpr (UPDATE) {
-      const float nx = fmaf(acc[g], gp(A.invCol)[(size_t)col * d.ZN + z], vcol[z]);
-      vcol[z] = nx;
-      if (A.shadow)
-        gp(A.shadow)[(size_t)col * d.ZN + z] = to_bf16_dither(nx, (size_t)col * d.ZN + z);
-    } else {
-      vcol[z] = acc[g];
+    if (z < d.ZN) bp_store<UPDATE>(A, vcol, col, z, acc[g]);
+  }
+}
+
+// Windowed form of bp_ref_body: one wave per voxel column over all of its z.  On a
+// helical scan a view's cone covers a window of a few dozen voxels of a column, which
+// the fixed lane -> z groups above cover with one 64-lane pass per group it touches
+// (about half the lanes idle).  Here the lanes take the view's own window, z0 + lane,
+// z0 + 64 + lane, ... (z0, z1 from the row shadow, widened by 2 voxels; the per-voxel
+// test decides), and the column's sums live in LDS (lacc: ZN floats of this wave), each
+// updated by one lane per view, views in order.  Per-voxel arithmetic as bp_ref_body.
+template <bool UPDATE, typename T>
+__device__ __forceinline__ void bp_win_body(const BpArgs<T>& A, int col, float* lacc) {
+  const Dims& d = A.d;
+  const BpConsts64& c64 = A.c64;
+  const int lane = threadIdx.x & 63;
+  if (col >= d.XN * d.YN) return;
+  const int y = col / d.XN, x = col - y * d.XN;
+  CTR_GLOBAL float* vcol = gp(A.vol) + (size_t)col * d.ZN;
+  if (gp(A.mask)[col] != 1) {
+    if constexpr (!UPDATE) {
+      for (int z = lane; z < d.ZN; z += 64) vcol[z] = 0.f;
+    }
+    return;
+  }
+  for (int z = lane; z < d.ZN; z += 64) lacc[z] = 0.f;
+  const double Px = ((double)x - c64.ocx) * c64.dx;
+  const double Py = ((double)y - c64.ocy) * c64.dx;
+  const float Pxf = (float)Px, Pyf = (float)Py;
+  const float DNVh = (float)d.DNV - 0.5f;
+  const float ZNf = (float)d.ZN;
+  const float idz = 1.f / d.dz;
+  const int DNV = d.DNV;
+  const uint32_t frame = (uint32_t)(d.DNU * d.DNV);
+  for (int k0 = 0; k0 < A.vc; k0 += 64) {
+    const int kl = k0 + lane;
+    BpView q{};
+    int q_z0 = 0, q_z1 = -1;
+    if (kl < A.vc) {
+      q = bp_view(A, A.vb + kl * A.vs, Px, Py, Pxf, Pyf);
+      // voxels with fmaf(dZ, rlS, cB) > -0.5 and fmaf(dZ, rlS, cA) < DNV - 0.5
+      const float ea = (-0.5f - q.cB) / q.rlS, eb = (DNVh - q.cA) / q.rlS;
+      const float za = fminf(fmaxf((fminf(ea, eb) + q.sz) * idz + d.ocz, -4.f), ZNf + 4.f);
+      const float zb = fminf(fmaxf((fmaxf(ea, eb) + q.sz) * idz + d.ocz, -4.f), ZNf + 4.f);
+      q_z0 = max((int)floorf(za) - 2, 0);
+      q_z1 = min((int)ceilf(zb) + 2, d.ZN - 1);
+    }
+    uint64_t live = __ballot(q.uvis && q_z0 <= q_z1);
+    while (live) {
+      const int j = __builtin_ctzll(live);
+      live &= live - 1;
+      const int k = k0 + j;
+      const float rlS = rdl(q.rlS, j), cA = rdl(q.cA, j), cB = rdl(q.cB, j);
+      const float vsz = rdl(q.sz, j), l2f = rdl(q.l2, j), cws = rdl(q.cws, j);
+      const float wu0 = rdl(q.wu0, j), fh = rdl(q.fh, j);
+      const int u0 = __builtin_amdgcn_readlane(q.u0, j), nu = __builtin_amdgcn_readlane(q.nu, j);
+      const int z0 = __builtin_amdgcn_readlane(q_z0, j), z1 = __builtin_amdgcn_readlane(q_z1, j);
+      const Buf<T> pp(A.prj + (size_t)k * frame, frame);
+      for (int zz = z0; zz <= z1; zz += 64) {
+        const int z = zz + lane;
+        const float dZ = fmaf((float)z - d.ocz, d.dz, -vsz);
+        const float vlo = fmaxf(fmaf(dZ, rlS, cA), -0.5f);
+        const float vhi = fminf(fmaf(dZ, rlS, cB), DNVh);
+        const bool vok = z <= z1 && vlo < vhi;
+        if (!vok) continue;
+        const float cosVal = cws * __builtin_amdgcn_sqrtf(fmaf(dZ, dZ, l2f));
+        lacc[z] = fmaf(cosVal, bp_cell_sum(pp, u0, nu, wu0, fh, vlo, vhi, DNV), lacc[z]);
+      }
     }
   }
+  for (int z = lane; z < d.ZN; z += 64) bp_store<UPDATE>(A, vcol, col, z, lacc[z]);
 }
 
 template <bool UPDATE, int ZG, typename T>
 __global__ __launch_bounds__(kBlock) void bp_ref_kernel(BpArgs<T> A) {
   const int col = blockIdx.x * kWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   bp_ref_body<UPDATE, ZG, T>(A, col, blockIdx.y);
+}
+
+// Windowed reference back projection; dynamic LDS: kWaves * ZN floats.
+template <bool UPDATE, typename T>
+__global__ __launch_bounds__(kBlock) void bp_win_kernel(BpArgs<T> A) {
+  extern __shared__ float lds_acc[];
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  bp_win_body<UPDATE, T>(A, blockIdx.x * kWaves + wid, lds_acc + wid * A.d.ZN);
+}
+
+// Batched windowed reference back projection: flat work list of (scan, block of 4
+// columns); dynamic LDS: kWaves * zmax floats (zmax: the tallest column of the batch).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bp_batch_win_kernel(const BpBatchItem<T>* __restrict__ D,
+                                                              int nscan, int total, int sched,
+                                                              int zmax) {
+  extern __shared__ float lds_acc[];
+  int i0, step, i1;
+  batch_range(D, nscan, total, sched, i0, step, i1);
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int s = 0;
+  for (int item = i0; item < i1; item += step) {
+    if (sched == 1) s = 0;
+    while (s + 1 < nscan && D[s + 1].begin <= item) ++s;
+    const BpBatchItem<T>& B = D[s];
+    bp_win_body<true, T>(B.a, (item - B.begin) * kWaves + wid, lds_acc + wid * zmax);
+  }
 }
 
 // Batched persistent reference back projection: flat work list of (scan, z block,
@@ -812,6 +938,15 @@ int launch_bp(const Plan& P, const T* prj, float* vol, int vb, int vs, int vc, i
   }
   const BpArgs<T> A = bp_args<T>(P, prj, vol, vb, vs, vc, accumulate ? invCol : nullptr,
                                  accumulate ? shadow : nullptr);
+  if (P.d.ZN <= kWinMaxZ) {
+    const dim3 wgrid((P.d.XN * P.d.YN + kWaves - 1) / kWaves);
+    const size_t lds = (size_t)kWaves * P.d.ZN * sizeof(float);
+    if (accumulate)
+      hipLaunchKernelGGL((bp_win_kernel<true, T>), wgrid, dim3(kBlock), lds, st, A);
+    else
+      hipLaunchKernelGGL((bp_win_kernel<false, T>), wgrid, dim3(kBlock), lds, st, A);
+    return check_launch("bp_win_kernel");
+  }
 #define CTR_BPREF(UPD, ZGV) \
   hipLaunchKernelGGL((bp_ref_kernel<UPD, ZGV, T>), rgrid, dim3(kBlock), 0, st, A)
   if (accumulate) {
@@ -869,6 +1004,18 @@ int launch_bp_batch(const BpBatchItem<T>* items, int nscan, int total, int zg, i
   return check_launch("bp_batch_kernel");
 }
 
+template <typename T>
+int launch_bp_batch_win(const BpBatchItem<T>* items, int nscan, int total, int zmax, int grid,
+                        int strided, hipStream_t st) {
+  if (total <= 0) return CTR_OK;
+  if (zmax < 1 || zmax > kWinMaxZ) return fail(CTR_ERR_ARG, "bp_batch_win: column height");
+  const int sched = grid <= 0 ? 2 : (strided ? 1 : 0);
+  grid = grid <= 0 ? total : std::max(1, std::min(grid, total));
+  hipLaunchKernelGGL((bp_batch_win_kernel<T>), dim3(grid), dim3(kBlock),
+                     (size_t)kWaves * zmax * sizeof(float), st, items, nscan, total, sched, zmax);
+  return check_launch("bp_batch_win_kernel");
+}
+
 int launch_reduce_batch(const ReduceItem* items, int nscan, hipStream_t st) {
   hipLaunchKernelGGL(reduce_batch_kernel, dim3(nscan), dim3(kBlock), 0, st, items);
   return check_launch("reduce_batch_kernel");
@@ -910,6 +1057,10 @@ template int launch_fp_batch<uint16_t>(const FpBatchItem<uint16_t>*, int, int, b
                                        hipStream_t);
 template int launch_bp_batch<float>(const BpBatchItem<float>*, int, int, int, int, int,
                                     hipStream_t);
+template int launch_bp_batch_win<float>(const BpBatchItem<float>*, int, int, int, int, int,
+                                        hipStream_t);
+template int launch_bp_batch_win<uint16_t>(const BpBatchItem<uint16_t>*, int, int, int, int,
+                                           int, hipStream_t);
 template int launch_bp_batch<uint16_t>(const BpBatchItem<uint16_t>*, int, int, int, int, int,
                                        hipStream_t);
 template int launch_scale_matched<float>(const Plan&, const float*, float*, int, int, int,

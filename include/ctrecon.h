@@ -228,7 +228,7 @@ int ctr_residual(ctr_plan* plan, const float* d_x, const float* d_prj, double* o
  * launch shapes on the batch (persistent grid sizes, forward work mapping, z groups per
  * lane of the back projector, contiguous / interleaved / one-per-item work assignment,
  * chosen by timing); each subset step then runs as one
- * forward, one reduction and one back-projection launch per z-group class over all
+ * forward, one reduction and one back-projection launch per back-projection class over all
  * scans, and a whole sweep is captured once into a HIP graph and replayed.  Reference back projector
  * only. */
 typedef struct ctr_batch ctr_batch;
@@ -236,7 +236,8 @@ typedef struct ctr_batch_info {
   int fp_colw;        /* forward mapping: 1 = one wave per detector column, 0 = per ray */
   int fp_grid;        /* persistent workgroups of the forward launch (0: one per item) */
   int fp_strided;     /* 1: interleaved item ranges, 0: contiguous */
-  int bp_zg;          /* z groups per lane of the back projector (0: each scan's ceil(ZN/64)) */
+  int bp_zg;          /* back projector: z groups per lane 1..4, 0: each scan's ceil(ZN/64),
+                         5: windowed (one wave per column over all z, sums in LDS) */
   int bp_grid;        /* persistent workgroups of the back-projection launch (0: one per item) */
   int bp_strided;     /* 1: interleaved item ranges, 0: contiguous */
   double setup_ms;    /* row / column sums of all scans */

@@ -79,15 +79,20 @@ def test_batch_equals_per_scan(storage, gpu_plan_factory):
     plans, xs, bs = _setup(storage)
     ref = _per_scan(plans, bs, 3)
     B = Batch(plans, xs, bs, SUBSETS)
-    B.set_tuning(True, 512, 1, 512)  # the per-scan launch shapes (all ZN <= 64)
-    res, ms = B.sweeps(3)
-    for i, (x_ref, r_ref) in enumerate(ref):
-        x = xs[i].get()
-        _say(f"  scan {i}: max |batch - per-scan| = {np.max(np.abs(x - x_ref)):.3e}; "
-             f"residuals {res[:, i]} vs {r_ref}")
-        assert np.array_equal(x, x_ref)
-        # the FP64 residual partials are summed in another slot order
-        assert np.allclose(res[:, i], r_ref, rtol=1e-12, atol=0)
+    # the per-scan launch shapes: z-group back projector (all ZN <= 64), then the
+    # windowed one; both compute every voxel's sum with the same arithmetic in view order
+    for zg in (1, 5):
+        for x in xs:
+            x.zero()
+        B.set_tuning(True, 512, zg, 512)
+        res, ms = B.sweeps(3)
+        for i, (x_ref, r_ref) in enumerate(ref):
+            x = xs[i].get()
+            _say(f"  zg {zg} scan {i}: max |batch - per-scan| = "
+                 f"{np.max(np.abs(x - x_ref)):.3e}; residuals {res[:, i]} vs {r_ref}")
+            assert np.array_equal(x, x_ref)
+            # the FP64 residual partials are summed in another slot order
+            assert np.allclose(res[:, i], r_ref, rtol=1e-12, atol=0)
 
 
 def test_graph_replay_equals_launches(gpu_plan_factory):
@@ -128,7 +133,8 @@ def test_tuned_batch_matches_per_scan(gpu_plan_factory):
     for colw, zg, grid, strided in [(True, 0, 0, False), (True, 1, 1024, False),
                                     (False, 2, 2048, True),
                                     (True, 4, 0, False), (False, 1, 0, True),
-                                    (True, 2, 512, True)]:
+                                    (True, 2, 512, True), (True, 5, 0, False),
+                                    (False, 5, 1024, True), (True, 5, 512, False)]:
         for x in xs:
             x.zero()
         B.set_tuning(colw, grid, zg, grid, strided, strided)
