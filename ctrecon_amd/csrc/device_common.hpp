@@ -81,6 +81,9 @@ __device__ __forceinline__ void st(CTR_GLOBAL uint16_t* p, size_t i, float v) {
 // num_records return 0 instead of touching memory outside the array.
 template <typename T>
 struct Buf;
+//
+// tap3(o, t1, t2, ...): elements o, o + t1, o + t2 with t1, t2 in {0, 1, 2} (taps of one
+// footprint along a contiguous axis; an unused tap repeats element o).
 template <>
 struct Buf<float> {
   __amdgpu_buffer_rsrc_t r;
@@ -89,6 +92,12 @@ struct Buf<float> {
                                             0x00020000)) {}
   __device__ __forceinline__ float operator[](int i) const {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, i * 4, 0, 0));
+  }
+  __device__ __forceinline__ void tap3(int o, int t1, int t2, float& a0, float& a1,
+                                       float& a2) const {
+    a0 = (*this)[o];
+    a1 = (*this)[o + t1];
+    a2 = (*this)[o + t2];
   }
 };
 template <>
@@ -99,6 +108,14 @@ struct Buf<uint16_t> {
                                             0x00020000)) {}
   __device__ __forceinline__ float operator[](int i) const {
     return __uint_as_float((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, i * 2, 0, 0) << 16);
+  }
+  // 16-bit loads (two aligned 32-bit loads with half-word selects measured slower: the
+  // projectors are VALU-bound)
+  __device__ __forceinline__ void tap3(int o, int t1, int t2, float& a0, float& a1,
+                                       float& a2) const {
+    a0 = (*this)[o];
+    a1 = (*this)[o + t1];
+    a2 = (*this)[o + t2];
   }
 };
 using BufF32 = Buf<float>;

@@ -63,6 +63,28 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* slot) {
 // T: storage type of vol and out (float, or uint16_t for BF16); b and invRow are FP32
 // and the ray sum is accumulated in FP32.
 // ---------------------------------------------------------------------------
+// NJ in-plane cells x 3 z taps of a slice footprint (cell stride cS, mask stride mC):
+// all 3 NJ loads are issued before the first is used, then the taps are combined in the
+// order of the general loop in fp_body.
+template <int NJ, typename TV>
+__device__ __forceinline__ float fp_cells(const Buf<TV>& V, const CTR_GLOBAL float* maskf, int o,
+                                          int mo, int cS, int mC, int t1, int t2, float wz0,
+                                          float wz1, float wz2, float wj0, float wjl) {
+  float a[NJ][3];
+#pragma unroll
+  for (int ij = 0; ij < NJ; ++ij) V.tap3(o + ij * cS, t1, t2, a[ij][0], a[ij][1], a[ij][2]);
+  float s = 0.f;
+#pragma unroll
+  for (int ij = 0; ij < NJ; ++ij) {
+    const float wy = ((ij == 0) ? wj0 : (ij == NJ - 1 ? wjl : 1.f)) * maskf[mo + ij * mC];
+    float g = wz0 * a[ij][0];
+    g = fmaf(wz1, a[ij][1], g);
+    g = fmaf(wz2, a[ij][2], g);
+    s = fmaf(wy, g, s);
+  }
+  return s;
+}
+
 template <bool FUSED, bool COLW, typename TV, typename TO>
 __device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk, double* slot) {
   const Dims& d = A.d;
@@ -113,14 +135,24 @@ __device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk
       const int t1 = nk > 1 ? 1 : 0, t2 = nk > 2 ? 2 : 0;
       float s = 0.f;
       if (!__any(nk > 3)) {
-        for (int ij = 0; ij < nj; ++ij) {
-          const float wy = ((ij == 0) ? wj0 : (ij == nj - 1 ? wjl : 1.f)) * maskf[mo];
-          float g = wz0 * V[o];
-          g = fmaf(wz1, V[o + t1], g);
-          g = fmaf(wz2, V[o + t2], g);
-          s = fmaf(wy, g, s);
-          o += cS;
-          mo += mC;
+        if (COLW && sizeof(TV) == 4 && nj <= 3) {
+          // nj is uniform here: straight-line code per cell count, all loads first (FP32
+          // volumes; with BF16 volumes the sub-dword gathers run faster one cell at a time)
+          switch (nj) {
+            case 1: s = fp_cells<1>(V, maskf, o, mo, cS, mC, t1, t2, wz0, wz1, wz2, wj0, wjl); break;
+            case 2: s = fp_cells<2>(V, maskf, o, mo, cS, mC, t1, t2, wz0, wz1, wz2, wj0, wjl); break;
+            default: s = fp_cells<3>(V, maskf, o, mo, cS, mC, t1, t2, wz0, wz1, wz2, wj0, wjl); break;
+          }
+        } else {
+          for (int ij = 0; ij < nj; ++ij) {
+            const float wy = ((ij == 0) ? wj0 : (ij == nj - 1 ? wjl : 1.f)) * maskf[mo];
+            float g = wz0 * V[o];
+            g = fmaf(wz1, V[o + t1], g);
+            g = fmaf(wz2, V[o + t2], g);
+            s = fmaf(wy, g, s);
+            o += cS;
+            mo += mC;
+          }
         }
       } else {
         for (int ij = 0; ij < nj; ++ij) {
@@ -342,6 +374,27 @@ __device__ __forceinline__ BpView bp_view(const BpArgs<T>& A, int p, double Px, 
   return r;
 }
 
+// NU columns x 3 row taps of a shadow: all 3 NU loads are issued before the first is
+// used (NU is the uniform column count, so every case is straight-line code), then the
+// taps are combined in the order of the general loop below.
+template <int NU, typename T>
+__device__ __forceinline__ float bp_cells(const Buf<T>& pp, int o, int DNV, int t1, int t2,
+                                          float wv0, float wv1, float wv2, float wu0, float fh) {
+  float a[NU][3];
+#pragma unroll
+  for (int iu = 0; iu < NU; ++iu) pp.tap3(o + iu * DNV, t1, t2, a[iu][0], a[iu][1], a[iu][2]);
+  float s = 0.f;
+#pragma unroll
+  for (int iu = 0; iu < NU; ++iu) {
+    const float wu = (iu == 0) ? wu0 : (iu == NU - 1 ? fh : 1.f);
+    float gsum = wv0 * a[iu][0];
+    gsum = fmaf(wv1, a[iu][1], gsum);
+    gsum = fmaf(wv2, a[iu][2], gsum);
+    s = fmaf(wu, gsum, s);
+  }
+  return s;
+}
+
 // The voxel's shadow [vlo, vhi] (rows) x cells u0 .. u0 + nu - 1 integrated over the
 // piecewise-constant view pp (first / last column weights wu0, fh).
 template <typename T>
@@ -355,18 +408,14 @@ __device__ __forceinline__ float bp_cell_sum(const Buf<T>& pp, int u0, int nu, f
   if (nu <= 4 && nv <= 3) {
     const float wv1 = (nv == 2) ? wvl : (nv == 3 ? 1.f : 0.f);
     const float wv2 = (nv == 3) ? wvl : 0.f;
+    // unused taps re-read the first row of the column (weight 0)
     const int t1 = nv > 1 ? 1 : 0, t2 = nv > 2 ? 2 : 0;
-    int o = u0 * DNV + v0;
-#pragma unroll
-    for (int iu = 0; iu < 4; ++iu) {
-      if (iu < nu) {
-        const float wu = (iu == 0) ? wu0 : (iu == nu - 1 ? fh : 1.f);
-        float gsum = wv0 * pp[o];
-        gsum = fmaf(wv1, pp[o + t1], gsum);
-        gsum = fmaf(wv2, pp[o + t2], gsum);
-        s = fmaf(wu, gsum, s);
-        o += DNV;
-      }
+    const int o = u0 * DNV + v0;
+    switch (nu) {  // uniform over the wave
+      case 1: s = bp_cells<1>(pp, o, DNV, t1, t2, wv0, wv1, wv2, wu0, fh); break;
+      case 2: s = bp_cells<2>(pp, o, DNV, t1, t2, wv0, wv1, wv2, wu0, fh); break;
+      case 3: s = bp_cells<3>(pp, o, DNV, t1, t2, wv0, wv1, wv2, wu0, fh); break;
+      default: s = bp_cells<4>(pp, o, DNV, t1, t2, wv0, wv1, wv2, wu0, fh); break;
     }
   } else {
     for (int iu = 0; iu < nu; ++iu) {
