@@ -122,19 +122,21 @@ __device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk
       zh = fminf(zh, ZNh);
       const bool zok = zl < zh;
       if (!__any(zok)) return;
-      const int k0 = min((int)floorf(zl + 0.5f), ZN1);
-      const int nk = zok ? min((int)floorf(zh + 0.5f), ZN1) - k0 + 1 : 0;
-      float wz0, wzl;
-      edge_weights(zl, zh, k0, nk, wz0, wzl);
-      const float wz1 = (nk == 2) ? wzl : (nk >= 3 ? 1.f : 0.f);
-      const float wz2 = (nk == 3) ? wzl : 0.f;
-      if (nk == 0) wz0 = 0.f;
+      // first cell k0 = floor(zl + .5) and its upper edge e; with dk = zh - e the overlaps
+      // of cells k0, k0 + 1, k0 + 2 are min(zh, e) - zl, clamp(dk, 0, 1), clamp(dk - 1, 0, 1)
+      const float fk = floorf(zl + 0.5f);
+      const float ek = fk + 0.5f;
+      const int k0 = min((int)fk, ZN1);
+      const float dk = zok ? zh - ek : -1.f;
+      const float wz0 = zok ? fminf(zh, ek) - zl : 0.f;
+      const float wz1 = __builtin_amdgcn_fmed3f(dk, 0.f, 1.f);
+      const float wz2 = __builtin_amdgcn_fmed3f(dk - 1.f, 0.f, 1.f);
       int o = ii * sS + j0 * cS + k0;
       int mo = ii * mS + j0 * mC;
-      // unused taps re-read the first cell (weight 0), never a neighbouring column
-      const int t1 = nk > 1 ? 1 : 0, t2 = nk > 2 ? 2 : 0;
+      // taps of zero weight re-read the first cell, never a neighbouring column
+      const int t1 = dk > 0.f ? 1 : 0, t2 = dk > 1.f ? 2 : 0;
       float s = 0.f;
-      if (!__any(nk > 3)) {
+      if (!__any(dk > 2.f)) {
         if (COLW && sizeof(TV) == 4 && nj <= 3) {
           // nj is uniform here: straight-line code per cell count, all loads first (FP32
           // volumes; with BF16 volumes the sub-dword gathers run faster one cell at a time)
@@ -155,6 +157,9 @@ __device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk
           }
         }
       } else {
+        const int nk = zok ? min((int)floorf(zh + 0.5f), ZN1) - k0 + 1 : 0;
+        float wf, wzl;
+        edge_weights(zl, zh, k0, nk, wf, wzl);
         for (int ij = 0; ij < nj; ++ij) {
           const float wy = ((ij == 0) ? wj0 : (ij == nj - 1 ? wjl : 1.f)) * maskf[mo];
           float g = 0.f;
@@ -400,16 +405,20 @@ __device__ __forceinline__ float bp_cells(const Buf<T>& pp, int o, int DNV, int 
 template <typename T>
 __device__ __forceinline__ float bp_cell_sum(const Buf<T>& pp, int u0, int nu, float wu0,
                                              float fh, float vlo, float vhi, int DNV) {
-  const int v0 = (int)floorf(vlo + 0.5f);
-  const int nv = min((int)floorf(vhi + 0.5f), DNV - 1) - v0 + 1;
-  float wv0, wvl;
-  edge_weights(vlo, vhi, v0, nv, wv0, wvl);
+  // first row v0 = floor(vlo + .5) and its upper edge e; with d = vhi - e the overlaps of
+  // rows v0, v0 + 1, v0 + 2 are min(vhi, e) - vlo, clamp(d, 0, 1), clamp(d - 1, 0, 1)
+  // (the first / interior / last weights of edge_weights, without the row count)
+  const float fv = floorf(vlo + 0.5f);
+  const float e = fv + 0.5f;
+  const int v0 = (int)fv;
+  const float d = vhi - e;
   float s = 0.f;
-  if (nu <= 4 && nv <= 3) {
-    const float wv1 = (nv == 2) ? wvl : (nv == 3 ? 1.f : 0.f);
-    const float wv2 = (nv == 3) ? wvl : 0.f;
-    // unused taps re-read the first row of the column (weight 0)
-    const int t1 = nv > 1 ? 1 : 0, t2 = nv > 2 ? 2 : 0;
+  if (nu <= 4 && d <= 2.f) {
+    const float wv0 = fminf(vhi, e) - vlo;
+    const float wv1 = __builtin_amdgcn_fmed3f(d, 0.f, 1.f);
+    const float wv2 = __builtin_amdgcn_fmed3f(d - 1.f, 0.f, 1.f);
+    // taps of zero weight re-read the first row (never a row outside the shadow)
+    const int t1 = d > 0.f ? 1 : 0, t2 = d > 1.f ? 2 : 0;
     const int o = u0 * DNV + v0;
     switch (nu) {  // uniform over the wave
       case 1: s = bp_cells<1>(pp, o, DNV, t1, t2, wv0, wv1, wv2, wu0, fh); break;
@@ -418,6 +427,9 @@ __device__ __forceinline__ float bp_cell_sum(const Buf<T>& pp, int u0, int nu, f
       default: s = bp_cells<4>(pp, o, DNV, t1, t2, wv0, wv1, wv2, wu0, fh); break;
     }
   } else {
+    const int nv = min((int)floorf(vhi + 0.5f), DNV - 1) - v0 + 1;
+    float wv0, wvl;
+    edge_weights(vlo, vhi, v0, nv, wv0, wvl);
     for (int iu = 0; iu < nu; ++iu) {
       const float wu = (iu == 0) ? wu0 : (iu == nu - 1 ? fh : 1.f);
       const int o = (u0 + iu) * DNV;
