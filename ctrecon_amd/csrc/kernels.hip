@@ -571,12 +571,14 @@ __device__ __forceinline__ void bp_win_body(const BpArgs<T>& A, int col, float* 
     int q_z0 = 0, q_z1 = -1;
     if (kl < A.vc) {
       q = bp_view(A, A.vb + kl * A.vs, Px, Py, Pxf, Pyf);
-      // voxels with fmaf(dZ, rlS, cB) > -0.5 and fmaf(dZ, rlS, cA) < DNV - 0.5
+      // voxels with fmaf(dZ, rlS, cB) > -0.5 and fmaf(dZ, rlS, cA) < DNV - 0.5 lie in the
+      // open interval (za, zb) of voxel indices; za, zb are within ~1e-4 of it (FP32), so
+      // the integers inside (za - m, zb + m), m = 1e-2, include every one of them
       const float ea = (-0.5f - q.cB) / q.rlS, eb = (DNVh - q.cA) / q.rlS;
       const float za = fminf(fmaxf((fminf(ea, eb) + q.sz) * idz + d.ocz, -4.f), ZNf + 4.f);
       const float zb = fminf(fmaxf((fmaxf(ea, eb) + q.sz) * idz + d.ocz, -4.f), ZNf + 4.f);
-      q_z0 = max((int)floorf(za) - 2, 0);
-      q_z1 = min((int)ceilf(zb) + 2, d.ZN - 1);
+      q_z0 = max((int)floorf(za - 1e-2f) + 1, 0);
+      q_z1 = min((int)ceilf(zb + 1e-2f) - 1, d.ZN - 1);
     }
     uint64_t live = __ballot(q.uvis && q_z0 <= q_z1);
     while (live) {
