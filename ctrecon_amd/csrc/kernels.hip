@@ -66,20 +66,29 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* slot) {
 // NJ in-plane cells x 3 z taps of a slice footprint (cell stride cS, mask stride mC):
 // all 3 NJ loads are issued before the first is used, then the taps are combined in the
 // order of the general loop in fp_body.
-template <int NJ, typename TV>
+// NT = 2 when no lane needs a third z cell (wz2 == 0 everywhere: the term it drops is an
+// exact zero).
+template <int NJ, int NT, typename TV>
 __device__ __forceinline__ float fp_cells(const Buf<TV>& V, const CTR_GLOBAL float* maskf, int o,
                                           int mo, int cS, int mC, int t1, int t2, float wz0,
                                           float wz1, float wz2, float wj0, float wjl) {
   float a[NJ][3];
 #pragma unroll
-  for (int ij = 0; ij < NJ; ++ij) V.tap3(o + ij * cS, t1, t2, a[ij][0], a[ij][1], a[ij][2]);
+  for (int ij = 0; ij < NJ; ++ij) {
+    if constexpr (NT == 3) {
+      V.tap3(o + ij * cS, t1, t2, a[ij][0], a[ij][1], a[ij][2]);
+    } else {
+      a[ij][0] = V[o + ij * cS];
+      a[ij][1] = V[o + ij * cS + t1];
+    }
+  }
   float s = 0.f;
 #pragma unroll
   for (int ij = 0; ij < NJ; ++ij) {
     const float wy = ((ij == 0) ? wj0 : (ij == NJ - 1 ? wjl : 1.f)) * maskf[mo + ij * mC];
     float g = wz0 * a[ij][0];
     g = fmaf(wz1, a[ij][1], g);
-    g = fmaf(wz2, a[ij][2], g);
+    if constexpr (NT == 3) g = fmaf(wz2, a[ij][2], g);
     s = fmaf(wy, g, s);
   }
   return s;
@@ -140,10 +149,18 @@ __device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk
         if (COLW && sizeof(TV) == 4 && nj <= 3) {
           // nj is uniform here: straight-line code per cell count, all loads first (FP32
           // volumes; with BF16 volumes the sub-dword gathers run faster one cell at a time)
-          switch (nj) {
-            case 1: s = fp_cells<1>(V, maskf, o, mo, cS, mC, t1, t2, wz0, wz1, wz2, wj0, wjl); break;
-            case 2: s = fp_cells<2>(V, maskf, o, mo, cS, mC, t1, t2, wz0, wz1, wz2, wj0, wjl); break;
-            default: s = fp_cells<3>(V, maskf, o, mo, cS, mC, t1, t2, wz0, wz1, wz2, wj0, wjl); break;
+          if (!__any(dk > 1.f)) {  // at most two z cells in every lane's footprint
+            switch (nj) {
+              case 1: s = fp_cells<1, 2>(V, maskf, o, mo, cS, mC, t1, t2, wz0, wz1, wz2, wj0, wjl); break;
+              case 2: s = fp_cells<2, 2>(V, maskf, o, mo, cS, mC, t1, t2, wz0, wz1, wz2, wj0, wjl); break;
+              default: s = fp_cells<3, 2>(V, maskf, o, mo, cS, mC, t1, t2, wz0, wz1, wz2, wj0, wjl); break;
+            }
+          } else {
+            switch (nj) {
+              case 1: s = fp_cells<1, 3>(V, maskf, o, mo, cS, mC, t1, t2, wz0, wz1, wz2, wj0, wjl); break;
+              case 2: s = fp_cells<2, 3>(V, maskf, o, mo, cS, mC, t1, t2, wz0, wz1, wz2, wj0, wjl); break;
+              default: s = fp_cells<3, 3>(V, maskf, o, mo, cS, mC, t1, t2, wz0, wz1, wz2, wj0, wjl); break;
+            }
           }
         } else {
           for (int ij = 0; ij < nj; ++ij) {
@@ -382,19 +399,28 @@ __device__ __forceinline__ BpView bp_view(const BpArgs<T>& A, int p, double Px, 
 // NU columns x 3 row taps of a shadow: all 3 NU loads are issued before the first is
 // used (NU is the uniform column count, so every case is straight-line code), then the
 // taps are combined in the order of the general loop below.
-template <int NU, typename T>
+// NT = 2 when no lane of the wave needs a third row (wv2 == 0 everywhere: the term it
+// drops is an exact zero).
+template <int NU, int NT, typename T>
 __device__ __forceinline__ float bp_cells(const Buf<T>& pp, int o, int DNV, int t1, int t2,
                                           float wv0, float wv1, float wv2, float wu0, float fh) {
   float a[NU][3];
 #pragma unroll
-  for (int iu = 0; iu < NU; ++iu) pp.tap3(o + iu * DNV, t1, t2, a[iu][0], a[iu][1], a[iu][2]);
+  for (int iu = 0; iu < NU; ++iu) {
+    if constexpr (NT == 3) {
+      pp.tap3(o + iu * DNV, t1, t2, a[iu][0], a[iu][1], a[iu][2]);
+    } else {
+      a[iu][0] = pp[o + iu * DNV];
+      a[iu][1] = pp[o + iu * DNV + t1];
+    }
+  }
   float s = 0.f;
 #pragma unroll
   for (int iu = 0; iu < NU; ++iu) {
     const float wu = (iu == 0) ? wu0 : (iu == NU - 1 ? fh : 1.f);
     float gsum = wv0 * a[iu][0];
     gsum = fmaf(wv1, a[iu][1], gsum);
-    gsum = fmaf(wv2, a[iu][2], gsum);
+    if constexpr (NT == 3) gsum = fmaf(wv2, a[iu][2], gsum);
     s = fmaf(wu, gsum, s);
   }
   return s;
@@ -420,11 +446,20 @@ __device__ __forceinline__ float bp_cell_sum(const Buf<T>& pp, int u0, int nu, f
     // taps of zero weight re-read the first row (never a row outside the shadow)
     const int t1 = d > 0.f ? 1 : 0, t2 = d > 1.f ? 2 : 0;
     const int o = u0 * DNV + v0;
-    switch (nu) {  // uniform over the wave
-      case 1: s = bp_cells<1>(pp, o, DNV, t1, t2, wv0, wv1, wv2, wu0, fh); break;
-      case 2: s = bp_cells<2>(pp, o, DNV, t1, t2, wv0, wv1, wv2, wu0, fh); break;
-      case 3: s = bp_cells<3>(pp, o, DNV, t1, t2, wv0, wv1, wv2, wu0, fh); break;
-      default: s = bp_cells<4>(pp, o, DNV, t1, t2, wv0, wv1, wv2, wu0, fh); break;
+    if (!__any(d > 1.f)) {  // at most two rows in every lane's shadow
+      switch (nu) {  // uniform over the wave
+        case 1: s = bp_cells<1, 2>(pp, o, DNV, t1, t2, wv0, wv1, wv2, wu0, fh); break;
+        case 2: s = bp_cells<2, 2>(pp, o, DNV, t1, t2, wv0, wv1, wv2, wu0, fh); break;
+        case 3: s = bp_cells<3, 2>(pp, o, DNV, t1, t2, wv0, wv1, wv2, wu0, fh); break;
+        default: s = bp_cells<4, 2>(pp, o, DNV, t1, t2, wv0, wv1, wv2, wu0, fh); break;
+      }
+    } else {
+      switch (nu) {
+        case 1: s = bp_cells<1, 3>(pp, o, DNV, t1, t2, wv0, wv1, wv2, wu0, fh); break;
+        case 2: s = bp_cells<2, 3>(pp, o, DNV, t1, t2, wv0, wv1, wv2, wu0, fh); break;
+        case 3: s = bp_cells<3, 3>(pp, o, DNV, t1, t2, wv0, wv1, wv2, wu0, fh); break;
+        default: s = bp_cells<4, 3>(pp, o, DNV, t1, t2, wv0, wv1, wv2, wu0, fh); break;
+      }
     }
   } else {
     const int nv = min((int)floorf(vhi + 0.5f), DNV - 1) - v0 + 1;
