@@ -15,7 +15,9 @@ target, with the row / column-sum set-up inside the timed region:
     warm-start network's predicted update, then plain OS-SART sweeps (with the
     fallback to plain OS-SART when the residual does not drop).
 ``value`` is the learned + BF16 time when trained weights are present
-(ctrecon_amd/weights/warm_start.npz), otherwise the plain FP32 time.
+(ctrecon_amd/weights/warm_start.npz; with ctrecon_amd/weights/warm_start_stage2.npz also
+present, the second-stage network acts after --after2 sweeps from the warm start),
+otherwise the plain FP32 time.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 
@@ -233,6 +235,10 @@ def main():
     ap.add_argument("--cpu-views", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", default="both", choices=["both", "plain", "learned"])
+    ap.add_argument("--weights2", default=os.path.join(ROOT, "ctrecon_amd", "weights",
+                                                       "warm_start_stage2.npz"),
+                    help="second-stage network (used when the file exists)")
+    ap.add_argument("--after2", type=int, default=3)
     ap.add_argument("--weights", default=os.path.join(ROOT, "ctrecon_amd", "weights",
                                                       "warm_start.npz"))
     ap.add_argument("--workload", default="clinical",
@@ -333,6 +339,7 @@ def main():
         from ctrecon_amd.unet import UNet
 
         net = UNet.from_file(args.weights)
+        net2 = UNet.from_file(args.weights2) if os.path.exists(args.weights2) else None
 
         def learned_leg(storage):
             plan.set_storage(storage)
@@ -347,7 +354,7 @@ def main():
             plan.synchronize()
             t0 = time.perf_counter()
             info = warm_start_os_sart(plan, net, d_b, d_x, args.subsets, args.target, max_iter,
-                                      args.bp)
+                                      args.bp, net2=net2, after2=args.after2)
             wall = time.perf_counter() - t0
             plan.set_storage("f32")
             r = np.array(info["residuals"])
@@ -357,12 +364,15 @@ def main():
                 "wall_s": wall,
                 "sweeps": int(info["sweeps"]),
                 "fallback_taken": bool(info["fallback"]),
+                "fallback2_taken": bool(info["fallback2"]),
                 "phase_s": {k: info[k] for k in ("setup_s", "pre_sweeps_s", "network_s",
                                                  "post_sweeps_s")},
                 "residual_sweep_final": float(r[-1]),
                 "residual_true_final": plan.residual(d_x, d_b),
                 "rmse_vs_truth_in_mask": rmse_of(d_x),
                 "weights": os.path.relpath(args.weights, ROOT),
+                "weights2": (os.path.relpath(args.weights2, ROOT) if net2 is not None
+                             else None),
             }
             log(f"learned + {storage}: {info['sweeps']} sweeps, wall {wall:.2f} s, "
                 f"residual {r[-1]:.3e}, fallback {info['fallback']}")

@@ -6,11 +6,16 @@ Pipeline (all projector and network work on the GPU through the C ABI):
   2. x3 = x2 + net(x2, delta), the network's predicted update;
   3. plain OS-SART from x3 until the data residual reaches the target.  If the first
      sweep from x3 sees a residual that is not below the one seen during sweep 2, the
-     iterate is reset to x2 and plain OS-SART continues from there (the fallback).
+     iterate is reset to x2 and plain OS-SART continues from there (the fallback);
+  4. optionally a second network at the start of the slow tail: after m further sweeps
+     (m = after2), x <- x_m + net2(x_m, x_m - x_(m-1)), with the same fallback.
 Training pairs come from plain OS-SART itself: on seeded phantoms the input is
-(x2, delta) and the target update is x_K - x2, x_K the K-sweep iterate.
+(x2, delta) and the target update is x_K - x2 (x_K the K-sweep iterate) or the phantom
+minus x2; second-stage pairs come from the warm-started solve at its m-th sweep.
 
     python -m ctrecon_amd.learned train --out weights.npz [--phantoms 64 --iters 50 ...]
+    python -m ctrecon_amd.learned train --out stage2.npz --stage2-from weights.npz \
+        --stage2-after 3 --targets truth ...
 """
 
 from __future__ import annotations
@@ -28,7 +33,8 @@ from .plan import DeviceArray, Plan, Sart
 from ._lib import check, lib
 from .unet import UNet, init_params
 
-__all__ = ["perlin3", "training_phantom", "make_pair", "train", "warm_start_os_sart"]
+__all__ = ["perlin3", "training_phantom", "make_pair", "make_pair_stage2", "train",
+           "warm_start_os_sart"]
 
 
 def _log(*a):
@@ -117,6 +123,45 @@ def make_pair(plan: Plan, vol, iters=50, osNum=20, kind="ref"):
             np.asarray(vol, np.float32) - x2)
 
 
+def _net_update(plan: Plan, net: UNet, d_x: DeviceArray, d_prev: DeviceArray,
+                d_keep: DeviceArray):
+    """d_keep <- d_x; d_x <- d_x + net(d_x, d_x - d_prev) (the update masked to the FOV)."""
+    shape = d_x.shape
+    delta = DeviceArray(shape)
+    _sub(delta, d_x, d_prev)
+    upd = DeviceArray(shape)
+    net.forward(d_x, delta, upd)
+    plan.mask_volume(upd)
+    d_keep.copy_from(d_x)
+    _axpy(d_x, upd, 1.0)
+
+
+def make_pair_stage2(plan: Plan, vol, net1: UNet, after=3, osNum=20, kind="ref"):
+    """(x_m, x_m - x_(m-1), None, vol - x_m) float32 host arrays: the warm-started solve on
+    b = A vol (two sweeps, net1, then `after` sweeps) at the point where the second
+    network acts."""
+    g = plan.g
+    d_b = DeviceArray(g.prj_shape)
+    plan.fp(DeviceArray.from_host(vol), d_b)
+    S = Sart(plan, d_b, osNum, kind)
+    dx = DeviceArray(g.vol_shape)
+    dx.zero()
+    prev = DeviceArray(g.vol_shape)
+    keep = DeviceArray(g.vol_shape)
+    S.sweeps(dx, 1)
+    prev.copy_from(dx)
+    S.sweeps(dx, 1)
+    _net_update(plan, net1, dx, prev, keep)
+    if after > 1:
+        S.sweeps(dx, after - 1)
+    prev.copy_from(dx)
+    S.sweeps(dx, 1)
+    x = dx.get()
+    dprev = prev.get()
+    S.close()
+    return (x, x - dprev, None, np.asarray(vol, np.float32) - x)
+
+
 def _cosine(step, steps, lr):
     return lr * 0.5 * (1.0 + math.cos(math.pi * min(step, steps) / steps))
 
@@ -161,10 +206,12 @@ def evaluate(net: UNet, pairs) -> float:
 
 
 def warm_start_os_sart(plan: Plan, net: UNet, d_b: DeviceArray, d_x: DeviceArray, osNum=20,
-                       stop_resid=1e-3, max_sweeps=400, kind="ref", fallback=True, sart=None):
-    """The learned warm start followed by plain OS-SART (module docstring).  d_x receives
-    the result.  Returns a dict with the per-sweep residuals, the number of sweeps,
-    whether the fallback was taken, and the wall time of each phase (s)."""
+                       stop_resid=1e-3, max_sweeps=400, kind="ref", fallback=True, sart=None,
+                       net2: UNet = None, after2=3):
+    """The learned warm start followed by plain OS-SART (module docstring), with the
+    optional second network net2 after `after2` sweeps from x3.  d_x receives the result.
+    Returns a dict with the per-sweep residuals, the number of sweeps, whether the
+    fallback was taken (for either network), and the wall time of each phase (s)."""
     t0 = time.perf_counter()
     S = sart if sart is not None else Sart(plan, d_b, osNum, kind)
     t_setup = time.perf_counter() - t0
@@ -193,7 +240,37 @@ def warm_start_os_sart(plan: Plan, net: UNet, d_b: DeviceArray, d_x: DeviceArray
     else:
         res_after = list(r3)
     done = 3
-    if not (r3[-1] <= stop_resid and not took_fallback):
+    reached = r3[-1] <= stop_resid and not took_fallback
+    took_fallback2 = False
+    net2_s = 0.0
+    if net2 is not None and not took_fallback and not reached:
+        # after2 sweeps from x3 in all (one done), the last one's update kept
+        if after2 > 2:
+            r = S.sweeps(d_x, after2 - 2, stop_resid)[0]
+            res_after += list(r)
+            done += len(r)
+            reached = r[-1] <= stop_resid
+        if not reached:
+            prev = DeviceArray(shape)
+            prev.copy_from(d_x)
+            r = S.sweeps(d_x, 1, stop_resid)[0]
+            res_after += list(r)
+            done += 1
+            reached = r[-1] <= stop_resid
+            if not reached:
+                tn = time.perf_counter()
+                keep = DeviceArray(shape)
+                _net_update(plan, net2, d_x, prev, keep)
+                plan.synchronize()
+                net2_s = time.perf_counter() - tn
+                r5 = S.sweeps(d_x, 1, stop_resid)[0]
+                done += 1
+                res_after += list(r5)
+                if fallback and r5[0] >= res_after[-2]:
+                    d_x.copy_from(keep)
+                    took_fallback2 = True
+                reached = r5[-1] <= stop_resid and not took_fallback2
+    if not reached:
         r = S.sweeps(d_x, max_sweeps - done, stop_resid)[0]
         res_after += list(r)
         done += len(r)
@@ -201,8 +278,9 @@ def warm_start_os_sart(plan: Plan, net: UNet, d_b: DeviceArray, d_x: DeviceArray
     if sart is None:
         S.close()
     return {"residuals": res + res_after, "sweeps": done, "fallback": took_fallback,
-            "setup_s": t_setup, "pre_sweeps_s": t1 - t0 - t_setup, "network_s": t2 - t1,
-            "post_sweeps_s": t3 - t2, "total_s": t3 - t0}
+            "fallback2": took_fallback2, "setup_s": t_setup,
+            "pre_sweeps_s": t1 - t0 - t_setup, "network_s": t2 - t1 + net2_s,
+            "post_sweeps_s": t3 - t2 - net2_s, "total_s": t3 - t0}
 
 
 def _phantom_job(nxy, nz, seed):
@@ -211,7 +289,7 @@ def _phantom_job(nxy, nz, seed):
 
 
 def compare(plan: Plan, net: UNet, vol, target=1e-3, osNum=20, max_sweeps=400, kind="ref",
-            plain_storage=None, warm_storage=None):
+            plain_storage=None, warm_storage=None, net2: UNet = None, after2=3):
     """Plain OS-SART vs the warm start on b = A vol (storage modes default to the
     plan's): sweeps and wall times to the residual target."""
     g = plan.g
@@ -230,11 +308,13 @@ def compare(plan: Plan, net: UNet, vol, target=1e-3, osNum=20, max_sweeps=400, k
     t_plain = time.perf_counter() - t0
     S.close()
     plan.set_storage(warm_storage or keep)
-    info = warm_start_os_sart(plan, net, d_b, dx, osNum, target, max_sweeps, kind)
+    info = warm_start_os_sart(plan, net, d_b, dx, osNum, target, max_sweeps, kind, net2=net2,
+                              after2=after2)
     plan.set_storage(keep)
     return {"plain_sweeps": len(r_plain), "plain_s": t_plain, "plain_final": float(r_plain[-1]),
             "warm_sweeps": info["sweeps"], "warm_s": info["total_s"],
             "warm_final": float(info["residuals"][-1]), "fallback": info["fallback"],
+            "fallback2": info["fallback2"],
             "residual_after_warm_start": float(info["residuals"][2]),
             "residual_plain_sweep3": float(r_plain[2]) if len(r_plain) > 2 else None}
 
@@ -243,10 +323,12 @@ def _eval(a):
     g = G.dd3_geometry(G.config_training(a.nxy, a.nz), mask=G.fov_mask(a.nxy))
     plan = Plan(g, 0)
     net = UNet.from_file(a.weights)
+    net2 = UNet.from_file(a.weights2) if a.weights2 else None
     for seed in a.seeds:
         vol = PH.ellipsoid_mixture(g, seed=seed) if a.mixture else training_phantom(g, seed)
         r = compare(plan, net, vol, a.target, a.subsets, a.max_sweeps,
-                    plain_storage=a.plain_storage, warm_storage=a.warm_storage)
+                    plain_storage=a.plain_storage, warm_storage=a.warm_storage, net2=net2,
+                    after2=a.after2)
         print(f"seed {seed}: " + ", ".join(f"{k} {v:.4g}" if isinstance(v, float) else f"{k} {v}"
                                            for k, v in r.items()), flush=True)
 
@@ -275,6 +357,10 @@ def main(argv=None):
     tr.add_argument("--nz", type=int, default=128)
     tr.add_argument("--storage", default="bf16")
     tr.add_argument("--workers", type=int, default=12)
+    tr.add_argument("--stage2-from", default=None,
+                    help="first-stage weights: train the second-stage network on the "
+                         "warm-started solve instead")
+    tr.add_argument("--stage2-after", type=int, default=3)
     ev = sub.add_parser("eval", help="sweeps to a residual target, plain vs warm-started")
     ev.add_argument("--weights", required=True)
     ev.add_argument("--seeds", type=int, nargs="+", default=[20000, 20001])
@@ -287,6 +373,8 @@ def main(argv=None):
                     help="plain 40-ellipsoid mixtures (the benchmark's phantom recipe)")
     ev.add_argument("--plain-storage", default="f32")
     ev.add_argument("--warm-storage", default="bf16")
+    ev.add_argument("--weights2", default=None, help="second-stage weights")
+    ev.add_argument("--after2", type=int, default=3)
     a = ap.parse_args(argv)
     if a.cmd == "eval":
         return _eval(a)
@@ -304,8 +392,12 @@ def main(argv=None):
     plan = Plan(g, 0)
     plan.set_storage(a.storage)
     pairs = []
+    net1 = UNet.from_file(a.stage2_from) if a.stage2_from else None
     for i, vol in enumerate(vols):
-        pairs.append(make_pair(plan, vol, a.iters, a.subsets))
+        if net1 is not None:
+            pairs.append(make_pair_stage2(plan, vol, net1, a.stage2_after, a.subsets))
+        else:
+            pairs.append(make_pair(plan, vol, a.iters, a.subsets))
         _log(f"pair {i + 1}/{len(vols)} ({time.time() - t0:.0f} s)")
     del vols
     targets = [t.strip() for t in a.targets.split(",") if t.strip()]
@@ -325,7 +417,11 @@ def main(argv=None):
     for seed, make in evals:
         vol = make()
         for tname, net in nets.items():
-            r = compare(plan, net, vol, a.target, a.subsets)
+            if net1 is not None:
+                r = compare(plan, net1, vol, a.target, a.subsets, net2=net,
+                            after2=a.stage2_after)
+            else:
+                r = compare(plan, net, vol, a.target, a.subsets)
             print(f"eval seed {seed} target {tname}: " + ", ".join(
                 f"{k} {v:.4g}" if isinstance(v, float) else f"{k} {v}" for k, v in r.items()),
                 flush=True)
