@@ -51,3 +51,44 @@ def test_warm_start_saves_sweeps(gpu_plan_factory):
     assert not info["fallback"]
     assert info["sweeps"] < len(r_plain)
     assert rmse_w <= 1.01 * rmse_p
+
+
+WEIGHTS2 = os.path.join(os.path.dirname(WEIGHTS), "warm_start_stage2.npz")
+
+
+def test_second_stage_saves_sweeps(gpu_plan_factory):
+    """The second-stage network (applied after 3 sweeps from the warm start) against the
+    first stage alone, at clinical size on a held-out phantom of the benchmark's recipe
+    (40-ellipsoid mixture, seed 10002), BF16 storage, to rel. residual 1e-3: fewer sweeps,
+    no fallback, RMSE to the phantom within 1% of the first stage's or better."""
+    from ctrecon_amd import geometry as G
+    from ctrecon_amd import learned as L
+    from ctrecon_amd import phantom as PH
+    from ctrecon_amd.plan import DeviceArray, Plan, Sart
+    from ctrecon_amd.unet import UNet
+
+    print("[second stage] seed 10002, 512x512x256, target 1e-3", flush=True)
+    assert os.path.exists(WEIGHTS2), "trained weights are part of the product"
+    g = G.dd3_geometry(G.config_clinical(), mask=G.fov_mask(512))
+    vol = PH.ellipsoid_mixture(g, seed=10002)
+    plan = Plan(g, 0)
+    d_b = DeviceArray(g.prj_shape)
+    plan.fp(DeviceArray.from_host(vol), d_b)
+    plan.set_storage("bf16")
+    net, net2 = UNet.from_file(WEIGHTS), UNet.from_file(WEIGHTS2)
+    target = 1e-3
+    m = g.mask.reshape(g.YN, g.XN).astype(bool)
+    S = Sart(plan, d_b, 20)
+    dx = DeviceArray(g.vol_shape)
+    one = L.warm_start_os_sart(plan, net, d_b, dx, 20, target, 200, sart=S)
+    rmse1 = float(np.sqrt(np.mean((dx.get() - vol)[m] ** 2)))
+    two = L.warm_start_os_sart(plan, net, d_b, dx, 20, target, 200, sart=S, net2=net2,
+                               after2=3)
+    rmse2 = float(np.sqrt(np.mean((dx.get() - vol)[m] ** 2)))
+    print(f"  first stage: {one['sweeps']} sweeps (final {one['residuals'][-1]:.3e}, RMSE "
+          f"{rmse1:.4f}); both stages: {two['sweeps']} sweeps (final {two['residuals'][-1]:.3e}, "
+          f"RMSE {rmse2:.4f}, fallback {two['fallback']}/{two['fallback2']})", flush=True)
+    assert one["residuals"][-1] <= target and two["residuals"][-1] <= target
+    assert not two["fallback"] and not two["fallback2"]
+    assert two["sweeps"] < one["sweeps"]
+    assert rmse2 <= 1.01 * rmse1
