@@ -15,9 +15,9 @@ target, with the row / column-sum set-up inside the timed region:
     warm-start network's predicted update, then plain OS-SART sweeps (with the
     fallback to plain OS-SART when the residual does not drop).
 ``value`` is the learned + BF16 time when trained weights are present
-(ctrecon_amd/weights/warm_start.npz; with ctrecon_amd/weights/warm_start_stage2.npz also
-present, the second-stage network acts after --after2 sweeps from the warm start),
-otherwise the plain FP32 time.
+(ctrecon_amd/weights/warm_start.npz; the later-stage networks warm_start_stage2.npz and
+warm_start_stage3.npz act, when present, --after2 / --after3 sweeps after the previous
+network's update), otherwise the plain FP32 time.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 
@@ -239,6 +239,10 @@ def main():
                                                        "warm_start_stage2.npz"),
                     help="second-stage network (used when the file exists)")
     ap.add_argument("--after2", type=int, default=3)
+    ap.add_argument("--weights3", default=os.path.join(ROOT, "ctrecon_amd", "weights",
+                                                       "warm_start_stage3.npz"),
+                    help="third-stage network (used when it and the second stage exist)")
+    ap.add_argument("--after3", type=int, default=2)
     ap.add_argument("--weights", default=os.path.join(ROOT, "ctrecon_amd", "weights",
                                                       "warm_start.npz"))
     ap.add_argument("--workload", default="clinical",
@@ -339,7 +343,11 @@ def main():
         from ctrecon_amd.unet import UNet
 
         net = UNet.from_file(args.weights)
-        net2 = UNet.from_file(args.weights2) if os.path.exists(args.weights2) else None
+        stages = []
+        if os.path.exists(args.weights2):
+            stages.append((UNet.from_file(args.weights2), args.after2))
+            if os.path.exists(args.weights3):
+                stages.append((UNet.from_file(args.weights3), args.after3))
 
         def learned_leg(storage):
             plan.set_storage(storage)
@@ -354,7 +362,7 @@ def main():
             plan.synchronize()
             t0 = time.perf_counter()
             info = warm_start_os_sart(plan, net, d_b, d_x, args.subsets, args.target, max_iter,
-                                      args.bp, net2=net2, after2=args.after2)
+                                      args.bp, stages=stages)
             wall = time.perf_counter() - t0
             plan.set_storage("f32")
             r = np.array(info["residuals"])
@@ -364,15 +372,15 @@ def main():
                 "wall_s": wall,
                 "sweeps": int(info["sweeps"]),
                 "fallback_taken": bool(info["fallback"]),
-                "fallback2_taken": bool(info["fallback2"]),
+                "later_stage_fallbacks": info["stage_fallbacks"],
                 "phase_s": {k: info[k] for k in ("setup_s", "pre_sweeps_s", "network_s",
                                                  "post_sweeps_s")},
                 "residual_sweep_final": float(r[-1]),
                 "residual_true_final": plan.residual(d_x, d_b),
                 "rmse_vs_truth_in_mask": rmse_of(d_x),
                 "weights": os.path.relpath(args.weights, ROOT),
-                "weights2": (os.path.relpath(args.weights2, ROOT) if net2 is not None
-                             else None),
+                "later_stages": [os.path.relpath(f, ROOT) for f, _ in
+                                 zip([args.weights2, args.weights3], stages)],
             }
             log(f"learned + {storage}: {info['sweeps']} sweeps, wall {wall:.2f} s, "
                 f"residual {r[-1]:.3e}, fallback {info['fallback']}")

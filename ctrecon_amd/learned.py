@@ -7,15 +7,19 @@ Pipeline (all projector and network work on the GPU through the C ABI):
   3. plain OS-SART from x3 until the data residual reaches the target.  If the first
      sweep from x3 sees a residual that is not below the one seen during sweep 2, the
      iterate is reset to x2 and plain OS-SART continues from there (the fallback);
-  4. optionally a second network at the start of the slow tail: after m further sweeps
-     (m = after2), x <- x_m + net2(x_m, x_m - x_(m-1)), with the same fallback.
+  4. optionally later networks at the start of each slow tail: stage k acts m_k sweeps
+     after the previous network's update, x <- x_m + net_k(x_m, x_m - x_(m-1)), with the
+     same fallback.
 Training pairs come from plain OS-SART itself: on seeded phantoms the input is
 (x2, delta) and the target update is x_K - x2 (x_K the K-sweep iterate) or the phantom
-minus x2; second-stage pairs come from the warm-started solve at its m-th sweep.
+minus x2; the pairs of a later stage come from the warm-started solve (with the stages
+before it) at the point where it acts.
 
     python -m ctrecon_amd.learned train --out weights.npz [--phantoms 64 --iters 50 ...]
-    python -m ctrecon_amd.learned train --out stage2.npz --stage2-from weights.npz \
-        --stage2-after 3 --targets truth ...
+    python -m ctrecon_amd.learned train --out stage2.npz --stage-from weights.npz \
+        --stage-after 3 --targets truth ...
+    python -m ctrecon_amd.learned train --out stage3.npz \
+        --stage-from weights.npz,stage2.npz --stage-afters 3 --stage-after 2 ...
 """
 
 from __future__ import annotations
@@ -33,7 +37,7 @@ from .plan import DeviceArray, Plan, Sart
 from ._lib import check, lib
 from .unet import UNet, init_params
 
-__all__ = ["perlin3", "training_phantom", "make_pair", "make_pair_stage2", "train",
+__all__ = ["perlin3", "training_phantom", "make_pair", "make_pair_stage", "train",
            "warm_start_os_sart"]
 
 
@@ -136,10 +140,10 @@ def _net_update(plan: Plan, net: UNet, d_x: DeviceArray, d_prev: DeviceArray,
     _axpy(d_x, upd, 1.0)
 
 
-def make_pair_stage2(plan: Plan, vol, net1: UNet, after=3, osNum=20, kind="ref"):
+def make_pair_stage(plan: Plan, vol, net1: UNet, stages=(), after=3, osNum=20, kind="ref"):
     """(x_m, x_m - x_(m-1), None, vol - x_m) float32 host arrays: the warm-started solve on
-    b = A vol (two sweeps, net1, then `after` sweeps) at the point where the second
-    network acts."""
+    b = A vol (two sweeps, net1, then for each (net_k, m_k) of `stages` m_k sweeps and
+    net_k's update, then `after` sweeps) at the point where the next network acts."""
     g = plan.g
     d_b = DeviceArray(g.prj_shape)
     plan.fp(DeviceArray.from_host(vol), d_b)
@@ -152,10 +156,13 @@ def make_pair_stage2(plan: Plan, vol, net1: UNet, after=3, osNum=20, kind="ref")
     prev.copy_from(dx)
     S.sweeps(dx, 1)
     _net_update(plan, net1, dx, prev, keep)
-    if after > 1:
-        S.sweeps(dx, after - 1)
-    prev.copy_from(dx)
-    S.sweeps(dx, 1)
+    for netk, mk in list(stages) + [(None, after)]:
+        if mk > 1:
+            S.sweeps(dx, mk - 1)
+        prev.copy_from(dx)
+        S.sweeps(dx, 1)
+        if netk is not None:
+            _net_update(plan, netk, dx, prev, keep)
     x = dx.get()
     dprev = prev.get()
     S.close()
@@ -207,11 +214,13 @@ def evaluate(net: UNet, pairs) -> float:
 
 def warm_start_os_sart(plan: Plan, net: UNet, d_b: DeviceArray, d_x: DeviceArray, osNum=20,
                        stop_resid=1e-3, max_sweeps=400, kind="ref", fallback=True, sart=None,
-                       net2: UNet = None, after2=3):
-    """The learned warm start followed by plain OS-SART (module docstring), with the
-    optional second network net2 after `after2` sweeps from x3.  d_x receives the result.
-    Returns a dict with the per-sweep residuals, the number of sweeps, whether the
-    fallback was taken (for either network), and the wall time of each phase (s)."""
+                       stages=()):
+    """The learned warm start followed by plain OS-SART (module docstring); `stages` lists
+    the later networks as (net_k, m_k), net_k acting m_k sweeps after the previous
+    network's update.  d_x receives the result.  Returns a dict with the per-sweep
+    residuals, the number of sweeps, whether the first network's fallback was taken and,
+    per later stage, whether its fallback was taken (or None: not reached), and the wall
+    time of each phase (s)."""
     t0 = time.perf_counter()
     S = sart if sart is not None else Sart(plan, d_b, osNum, kind)
     t_setup = time.perf_counter() - t0
@@ -241,35 +250,39 @@ def warm_start_os_sart(plan: Plan, net: UNet, d_b: DeviceArray, d_x: DeviceArray
         res_after = list(r3)
     done = 3
     reached = r3[-1] <= stop_resid and not took_fallback
-    took_fallback2 = False
+    stage_fallbacks = [None] * len(stages)
     net2_s = 0.0
-    if net2 is not None and not took_fallback and not reached:
-        # after2 sweeps from x3 in all (one done), the last one's update kept
-        if after2 > 2:
-            r = S.sweeps(d_x, after2 - 2, stop_resid)[0]
+    prev = DeviceArray(shape)
+    keep = DeviceArray(shape)
+    for k, (netk, mk) in enumerate(stages):
+        if took_fallback or reached or any(stage_fallbacks[:k]):
+            break
+        # m_k sweeps from the previous update in all (one done), the last one's update kept
+        if mk > 2:
+            r = S.sweeps(d_x, mk - 2, stop_resid)[0]
             res_after += list(r)
             done += len(r)
             reached = r[-1] <= stop_resid
-        if not reached:
-            prev = DeviceArray(shape)
-            prev.copy_from(d_x)
-            r = S.sweeps(d_x, 1, stop_resid)[0]
-            res_after += list(r)
-            done += 1
-            reached = r[-1] <= stop_resid
-            if not reached:
-                tn = time.perf_counter()
-                keep = DeviceArray(shape)
-                _net_update(plan, net2, d_x, prev, keep)
-                plan.synchronize()
-                net2_s = time.perf_counter() - tn
-                r5 = S.sweeps(d_x, 1, stop_resid)[0]
-                done += 1
-                res_after += list(r5)
-                if fallback and r5[0] >= res_after[-2]:
-                    d_x.copy_from(keep)
-                    took_fallback2 = True
-                reached = r5[-1] <= stop_resid and not took_fallback2
+            if reached:
+                break
+        prev.copy_from(d_x)
+        r = S.sweeps(d_x, 1, stop_resid)[0]
+        res_after += list(r)
+        done += 1
+        reached = r[-1] <= stop_resid
+        if reached:
+            break
+        tn = time.perf_counter()
+        _net_update(plan, netk, d_x, prev, keep)
+        plan.synchronize()
+        net2_s += time.perf_counter() - tn
+        r5 = S.sweeps(d_x, 1, stop_resid)[0]
+        done += 1
+        res_after += list(r5)
+        stage_fallbacks[k] = bool(fallback and r5[0] >= res_after[-2])
+        if stage_fallbacks[k]:
+            d_x.copy_from(keep)
+        reached = r5[-1] <= stop_resid and not stage_fallbacks[k]
     if not reached:
         r = S.sweeps(d_x, max_sweeps - done, stop_resid)[0]
         res_after += list(r)
@@ -278,7 +291,7 @@ def warm_start_os_sart(plan: Plan, net: UNet, d_b: DeviceArray, d_x: DeviceArray
     if sart is None:
         S.close()
     return {"residuals": res + res_after, "sweeps": done, "fallback": took_fallback,
-            "fallback2": took_fallback2, "setup_s": t_setup,
+            "stage_fallbacks": stage_fallbacks, "setup_s": t_setup,
             "pre_sweeps_s": t1 - t0 - t_setup, "network_s": t2 - t1 + net2_s,
             "post_sweeps_s": t3 - t2 - net2_s, "total_s": t3 - t0}
 
@@ -289,7 +302,7 @@ def _phantom_job(nxy, nz, seed):
 
 
 def compare(plan: Plan, net: UNet, vol, target=1e-3, osNum=20, max_sweeps=400, kind="ref",
-            plain_storage=None, warm_storage=None, net2: UNet = None, after2=3):
+            plain_storage=None, warm_storage=None, stages=()):
     """Plain OS-SART vs the warm start on b = A vol (storage modes default to the
     plan's): sweeps and wall times to the residual target."""
     g = plan.g
@@ -308,13 +321,13 @@ def compare(plan: Plan, net: UNet, vol, target=1e-3, osNum=20, max_sweeps=400, k
     t_plain = time.perf_counter() - t0
     S.close()
     plan.set_storage(warm_storage or keep)
-    info = warm_start_os_sart(plan, net, d_b, dx, osNum, target, max_sweeps, kind, net2=net2,
-                              after2=after2)
+    info = warm_start_os_sart(plan, net, d_b, dx, osNum, target, max_sweeps, kind,
+                              stages=stages)
     plan.set_storage(keep)
     return {"plain_sweeps": len(r_plain), "plain_s": t_plain, "plain_final": float(r_plain[-1]),
             "warm_sweeps": info["sweeps"], "warm_s": info["total_s"],
             "warm_final": float(info["residuals"][-1]), "fallback": info["fallback"],
-            "fallback2": info["fallback2"],
+            "stage_fallbacks": info["stage_fallbacks"],
             "residual_after_warm_start": float(info["residuals"][2]),
             "residual_plain_sweep3": float(r_plain[2]) if len(r_plain) > 2 else None}
 
@@ -323,14 +336,22 @@ def _eval(a):
     g = G.dd3_geometry(G.config_training(a.nxy, a.nz), mask=G.fov_mask(a.nxy))
     plan = Plan(g, 0)
     net = UNet.from_file(a.weights)
-    net2 = UNet.from_file(a.weights2) if a.weights2 else None
+    stages = _stage_list(a.stages, a.afters)
     for seed in a.seeds:
         vol = PH.ellipsoid_mixture(g, seed=seed) if a.mixture else training_phantom(g, seed)
         r = compare(plan, net, vol, a.target, a.subsets, a.max_sweeps,
-                    plain_storage=a.plain_storage, warm_storage=a.warm_storage, net2=net2,
-                    after2=a.after2)
+                    plain_storage=a.plain_storage, warm_storage=a.warm_storage, stages=stages)
         print(f"seed {seed}: " + ", ".join(f"{k} {v:.4g}" if isinstance(v, float) else f"{k} {v}"
                                            for k, v in r.items()), flush=True)
+
+
+def _stage_list(files, afters):
+    """[(UNet, m_k)] from a comma list of weights files and one of sweep counts."""
+    fs = [f for f in (files or "").split(",") if f]
+    ms = [int(m) for m in (afters or "").split(",") if m]
+    if len(fs) != len(ms):
+        raise SystemExit("one sweep count per later-stage weights file")
+    return [(UNet.from_file(f), m) for f, m in zip(fs, ms)]
 
 
 def main(argv=None):
@@ -357,10 +378,14 @@ def main(argv=None):
     tr.add_argument("--nz", type=int, default=128)
     tr.add_argument("--storage", default="bf16")
     tr.add_argument("--workers", type=int, default=12)
-    tr.add_argument("--stage2-from", default=None,
-                    help="first-stage weights: train the second-stage network on the "
-                         "warm-started solve instead")
-    tr.add_argument("--stage2-after", type=int, default=3)
+    tr.add_argument("--stage-from", default=None,
+                    help="comma list: the first network's weights, then those of the later "
+                         "stages already trained; trains the next stage on the "
+                         "warm-started solve")
+    tr.add_argument("--stage-afters", default="",
+                    help="comma list: sweeps before each later stage already trained")
+    tr.add_argument("--stage-after", type=int, default=3,
+                    help="sweeps before the stage being trained")
     ev = sub.add_parser("eval", help="sweeps to a residual target, plain vs warm-started")
     ev.add_argument("--weights", required=True)
     ev.add_argument("--seeds", type=int, nargs="+", default=[20000, 20001])
@@ -373,8 +398,8 @@ def main(argv=None):
                     help="plain 40-ellipsoid mixtures (the benchmark's phantom recipe)")
     ev.add_argument("--plain-storage", default="f32")
     ev.add_argument("--warm-storage", default="bf16")
-    ev.add_argument("--weights2", default=None, help="second-stage weights")
-    ev.add_argument("--after2", type=int, default=3)
+    ev.add_argument("--stages", default="", help="comma list of later-stage weights")
+    ev.add_argument("--afters", default="", help="comma list: sweeps before each")
     a = ap.parse_args(argv)
     if a.cmd == "eval":
         return _eval(a)
@@ -392,10 +417,14 @@ def main(argv=None):
     plan = Plan(g, 0)
     plan.set_storage(a.storage)
     pairs = []
-    net1 = UNet.from_file(a.stage2_from) if a.stage2_from else None
+    net1, before = None, []
+    if a.stage_from:
+        files = a.stage_from.split(",")
+        net1 = UNet.from_file(files[0])
+        before = _stage_list(",".join(files[1:]), a.stage_afters)
     for i, vol in enumerate(vols):
         if net1 is not None:
-            pairs.append(make_pair_stage2(plan, vol, net1, a.stage2_after, a.subsets))
+            pairs.append(make_pair_stage(plan, vol, net1, before, a.stage_after, a.subsets))
         else:
             pairs.append(make_pair(plan, vol, a.iters, a.subsets))
         _log(f"pair {i + 1}/{len(vols)} ({time.time() - t0:.0f} s)")
@@ -418,8 +447,8 @@ def main(argv=None):
         vol = make()
         for tname, net in nets.items():
             if net1 is not None:
-                r = compare(plan, net1, vol, a.target, a.subsets, net2=net,
-                            after2=a.stage2_after)
+                r = compare(plan, net1, vol, a.target, a.subsets,
+                            stages=before + [(net, a.stage_after)])
             else:
                 r = compare(plan, net, vol, a.target, a.subsets)
             print(f"eval seed {seed} target {tname}: " + ", ".join(

@@ -82,13 +82,12 @@ def test_second_stage_saves_sweeps(gpu_plan_factory):
     dx = DeviceArray(g.vol_shape)
     one = L.warm_start_os_sart(plan, net, d_b, dx, 20, target, 200, sart=S)
     rmse1 = float(np.sqrt(np.mean((dx.get() - vol)[m] ** 2)))
-    two = L.warm_start_os_sart(plan, net, d_b, dx, 20, target, 200, sart=S, net2=net2,
-                               after2=3)
+    two = L.warm_start_os_sart(plan, net, d_b, dx, 20, target, 200, sart=S, stages=[(net2, 3)])
     rmse2 = float(np.sqrt(np.mean((dx.get() - vol)[m] ** 2)))
     print(f"  first stage: {one['sweeps']} sweeps (final {one['residuals'][-1]:.3e}, RMSE "
           f"{rmse1:.4f}); both stages: {two['sweeps']} sweeps (final {two['residuals'][-1]:.3e}, "
-          f"RMSE {rmse2:.4f}, fallback {two['fallback']}/{two['fallback2']})", flush=True)
+          f"RMSE {rmse2:.4f}, fallback {two['fallback']}/{two['stage_fallbacks']})", flush=True)
     assert one["residuals"][-1] <= target and two["residuals"][-1] <= target
-    assert not two["fallback"] and not two["fallback2"]
+    assert not two["fallback"] and two["stage_fallbacks"] == [False]
     assert two["sweeps"] < one["sweeps"]
     assert rmse2 <= 1.01 * rmse1
