@@ -55,6 +55,7 @@ struct Dims {
   float dx, dz;
   float beta0, dbeta_g;  // fan angle of boundary 0 and mean boundary spacing (adjoint guess)
   float B0, dB;          // row boundary index of row 0 and mean spacing (adjoint guess)
+  float dBmin;           // smallest row spacing of the FP32 boundary table
 };
 
 // Host copies used by host-side accounting (cone voxel counts).

@@ -826,6 +826,316 @@ __global__ __launch_bounds__(kBlock) void bp_matched_kernel(
   }
 }
 
+// Rows of one detector column meeting the z cell [zlo, zlo + 1] of a voxel, integrated
+// against the column's scaled sinogram yu with the forward projector's FP32 row
+// boundaries az + r Bz[v] (Bz from LDS); the first row from the uniform-spacing guess,
+// then corrected against the exact boundaries.
+__device__ __forceinline__ float matched_rows(const float* __restrict__ yu,
+                                              const float* __restrict__ sBz, int DNV, float r,
+                                              float az, float zlo, float B0, float dB) {
+  const float zhi = zlo + 1.0f;
+  int v = (int)fminf(fmaxf(floorf(((zlo - az) * __builtin_amdgcn_rcpf(r) - B0) / dB), 0.f),
+                     (float)(DNV - 1));
+  float zl = fmaf(r, sBz[v], az);
+  while (v > 0 && zl > zlo) {
+    --v;
+    zl = fmaf(r, sBz[v], az);
+  }
+  float zh = fmaf(r, sBz[v + 1], az);
+  while (v < DNV - 1 && zh <= zlo) {
+    ++v;
+    zl = zh;
+    zh = fmaf(r, sBz[v + 1], az);
+  }
+  float gsum = 0.f;
+  for (; v < DNV && zl < zhi;) {
+    const float wz = fminf(zh, zhi) - fmaxf(zl, zlo);
+    if (wz > 0.f) gsum = fmaf(wz, yu[v], gsum);
+    ++v;
+    if (v < DNV) {
+      zl = zh;
+      zh = fmaf(r, sBz[v + 1], az);
+    }
+  }
+  return gsum;
+}
+
+// The same sums as matched_rows for the (at most four) columns t < nu of one view, the
+// rows taken as a fixed count NR (at least the number of rows of spacing r_t dBmin that
+// can meet a cell of height 1): every column's first row is searched, then every
+// sinogram value is loaded, then the rows are summed in order, rows with no overlap or
+// past the detector adding nothing -- the sums and their order are those of matched_rows.
+// ig[t] = 1 / (r_t dB), g0 = -B0 / dB give the first-row guess.
+template <int NR>
+__device__ __forceinline__ float matched_cols(const float* __restrict__ yk,
+                                              const float* __restrict__ sBz, int DNV, int nu,
+                                              const float (&w)[4], const float (&r)[4],
+                                              const float (&ig)[4], float az, float g0,
+                                              float zlo, float a) {
+  const float zhi = zlo + 1.0f;
+  int v[4];
+  float zl[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    v[t] = 0;
+    zl[t] = 0.f;
+    if (t < nu) {
+      int vv = (int)fminf(fmaxf(floorf(fmaf(zlo - az, ig[t], g0)), 0.f), (float)(DNV - 1));
+      float l = fmaf(r[t], sBz[vv], az);
+      while (vv > 0 && l > zlo) {
+        --vv;
+        l = fmaf(r[t], sBz[vv], az);
+      }
+      float h = fmaf(r[t], sBz[vv + 1], az);
+      while (vv < DNV - 1 && h <= zlo) {
+        ++vv;
+        l = h;
+        h = fmaf(r[t], sBz[vv + 1], az);
+      }
+      v[t] = vv;
+      zl[t] = l;
+    }
+  }
+  float yv[4][NR];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+#pragma unroll
+    for (int i = 0; i < NR; ++i)
+      yv[t][i] = (t < nu) ? yk[(size_t)t * DNV + min(v[t] + i, DNV - 1)] : 0.f;
+  }
+#pragma unroll
+  for (int t = 0; t < 4; ++t) {
+    if (t < nu && w[t] != 0.f) {
+      float g = 0.f, lo = zl[t];
+#pragma unroll
+      for (int i = 0; i < NR; ++i) {
+        const int vi = v[t] + i;
+        const float hi = fmaf(r[t], sBz[min(vi + 1, DNV)], az);
+        const float wz = fminf(hi, zhi) - fmaxf(lo, zlo);
+        if (vi < DNV && wz > 0.f) g = fmaf(wz, yv[t][i], g);
+        lo = hi;
+      }
+      a = fmaf(w[t], g, a);
+    }
+  }
+  return a;
+}
+
+// Windowed matched adjoint: the transpose of fp_kernel as bp_matched_kernel computes it,
+// with one wave per voxel column over all of its z.  Per view (64 views at a time, one
+// per lane): the detector columns whose footprints on the voxel's slice meet its cell
+// (first column u0, count nu; for nu <= 4 the in-plane weights w_t = overlap / ((tr - tl)
+// r) and magnifications r_t), and the view's z-window of the column (the cells the
+// detector's z range at those magnifications reaches, widened by one).  The lanes then
+// take the window, each voxel summing w_t times its rows' integral over the columns in
+// order (the same sequence of operations per voxel as bp_matched_kernel), into the
+// column's LDS sums.  Dynamic LDS: Bz (DNV + 1 floats, 4-aligned), then kWaves * ZN.
+template <bool UPDATE>
+__global__ __launch_bounds__(kBlock) void bp_matched_win_kernel(
+    const float* __restrict__ yp, float* __restrict__ vol, const ViewP* __restrict__ views,
+    const float2* __restrict__ bnd, const float4* __restrict__ cols,
+    const float* __restrict__ Bz, const uint8_t* __restrict__ mask, Dims d, BpConsts c,
+    int vb, int vs, int vc, const float* __restrict__ invCol, uint16_t* __restrict__ shadow) {
+  extern __shared__ float lds_m[];
+  const int DNU = d.DNU, DNV = d.DNV;
+  const int nbz = (DNV + 1 + 3) & ~3;
+  float* sBz = lds_m;
+  for (int i = threadIdx.x; i <= DNV; i += kBlock) sBz[i] = Bz[i];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int col = blockIdx.x * kWaves + wid;
+  if (col >= d.XN * d.YN) return;
+  float* lacc = lds_m + nbz + wid * d.ZN;
+  const int y = col / d.XN, x = col - y * d.XN;
+  float* vcol = vol + (size_t)col * d.ZN;
+  const float m = (float)mask[col];
+  if (m == 0.f) {
+    if constexpr (!UPDATE) {
+      for (int z = lane; z < d.ZN; z += 64) vcol[z] = 0.f;
+    }
+    return;
+  }
+  for (int z = lane; z < d.ZN; z += 64) lacc[z] = 0.f;
+  const float Px = ((float)x - d.ocx) * d.dx;
+  const float Py = ((float)y - d.ocy) * d.dx;
+  const size_t frame = (size_t)DNU * DNV;
+  const float ZNf = (float)d.ZN;
+  for (int k0 = 0; k0 < vc; k0 += 64) {
+    const int kl = k0 + lane;
+    int q_u0 = 0, q_nu = 0, q_z0 = 0, q_z1 = -1;
+    float q_w[4] = {0.f, 0.f, 0.f, 0.f}, q_r[4] = {1.f, 1.f, 1.f, 1.f};
+    float q_ig[4] = {0.f, 0.f, 0.f, 0.f};
+    float q_az = 0.f;
+    int q_nr = 0;
+    if (kl < vc) {
+      const int p = vb + kl * vs;
+      const ViewP vp = views[p];
+      const int ii = vp.xmarch ? x : y;
+      const int j = vp.xmarch ? y : x;
+      const float fi = (float)ii;
+      const float2* bt = bnd + (size_t)p * (DNU + 1);
+      const float4* ctab = cols + (size_t)p * DNU;
+      const float dir = (float)vp.bdir;
+      const float cl = (vp.bdir > 0) ? (float)j - 0.5f : -((float)j + 0.5f);
+      const float ch = cl + 1.0f;
+      // first guess from the fan angle of the voxel centre, then walk to the first
+      // column whose footprint (same FP32 expressions as fp_kernel) reaches cl
+      const float dX = Px - vp.sx, dY = Py - vp.sy;
+      const float beta =
+          asinf((vp.sy * dX - vp.sx * dY) * c.inv_sid * rsqrtf(dX * dX + dY * dY));
+      int u = (int)fminf(fmaxf(floorf((beta - d.beta0) / d.dbeta_g), 0.f), (float)(DNU - 1));
+      float su = dir * fmaf(fi, bt[u].y, bt[u].x);
+      while (u > 0 && su > cl) {
+        --u;
+        su = dir * fmaf(fi, bt[u].y, bt[u].x);
+      }
+      float su1 = dir * fmaf(fi, bt[u + 1].y, bt[u + 1].x);
+      while (u < DNU - 1 && su1 <= cl) {
+        ++u;
+        su = su1;
+        su1 = dir * fmaf(fi, bt[u + 1].y, bt[u + 1].x);
+      }
+      // the columns u .. while su < ch (zero-overlap ones included: they add nothing)
+      q_u0 = u;
+      float rmin = 3.4e38f, rmax = -3.4e38f;
+      int n = 0;
+      for (; u < DNU && su < ch;) {
+        const float4 ct = ctab[u];
+        const float r = fmaf(fi, ct.y, ct.x);
+        const float wy = fminf(su1, ch) - fmaxf(su, cl);
+        if (n < 4) {
+          q_w[n] = wy > 0.f ? wy * __builtin_amdgcn_rcpf((su1 - su) * r) : 0.f;
+          q_r[n] = r;
+          q_ig[n] = __builtin_amdgcn_rcpf(r * d.dB);
+        }
+        rmin = fminf(rmin, r);
+        rmax = fmaxf(rmax, r);
+        ++n;
+        ++u;
+        if (u < DNU) {
+          su = su1;
+          su1 = dir * fmaf(fi, bt[u + 1].y, bt[u + 1].x);
+        }
+      }
+      q_nu = n;
+      q_az = vp.az;
+      if (n > 0) {
+        // z cells [k - .5, k + .5] meeting [az + r Bz[0], az + r Bz[DNV]] for some r of
+        // the columns (r > 0 beyond the source)
+        const float lo = fminf(fmaf(rmin, Bz[0], vp.az), fmaf(rmax, Bz[0], vp.az));
+        const float hi = fmaxf(fmaf(rmin, Bz[DNV], vp.az), fmaf(rmax, Bz[DNV], vp.az));
+        const float za = fminf(fmaxf(lo - 0.5f, -4.f), ZNf + 4.f);
+        const float zb = fminf(fmaxf(hi + 0.5f, -4.f), ZNf + 4.f);
+        q_z0 = max((int)floorf(za) - 1, 0);
+        q_z1 = min((int)ceilf(zb) + 1, d.ZN - 1);
+        // rows of spacing >= h meeting a cell of height 1: at most ceil(1 / h) + 1; h is
+        // the smallest spacing rmin dBmin less the rounding of boundaries of size <= big
+        const float big = fmaxf(fmaxf(fabsf(lo), fabsf(hi)), 1.f);
+        const float h = rmin * d.dBmin * 0.99999f - 4.8e-7f * big;
+        q_nr = h > 0.125f ? (int)ceilf(__builtin_amdgcn_rcpf(h) * 1.0001f) + 1 : 99;
+      }
+    }
+    uint64_t live = __ballot(q_nu > 0 && q_z0 <= q_z1);
+    while (live) {
+      const int jl = __builtin_ctzll(live);
+      live &= live - 1;
+      const int k = k0 + jl;
+      const int u0 = __builtin_amdgcn_readlane(q_u0, jl), nu = __builtin_amdgcn_readlane(q_nu, jl);
+      const int z0 = __builtin_amdgcn_readlane(q_z0, jl), z1 = __builtin_amdgcn_readlane(q_z1, jl);
+      const float az = rdl(q_az, jl);
+      const int nr = __builtin_amdgcn_readlane(q_nr, jl);
+      const float* yk = yp + (size_t)k * frame + (size_t)u0 * DNV;
+      if (nu <= 4 && nr <= 5) {
+        float w[4], r[4], ig[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          w[t] = rdl(q_w[t], jl);
+          r[t] = rdl(q_r[t], jl);
+          ig[t] = rdl(q_ig[t], jl);
+        }
+        const float g0 = -d.B0 / d.dB;
+        for (int zz = z0; zz <= z1; zz += 64) {
+          const int z = zz + lane;
+          if (z > z1) continue;
+          const float zlo = (float)z - 0.5f;
+          const float a0 = lacc[z];
+          float a;
+          switch (nr) {
+            case 2: a = matched_cols<2>(yk, sBz, DNV, nu, w, r, ig, az, g0, zlo, a0); break;
+            case 3: a = matched_cols<3>(yk, sBz, DNV, nu, w, r, ig, az, g0, zlo, a0); break;
+            case 4: a = matched_cols<4>(yk, sBz, DNV, nu, w, r, ig, az, g0, zlo, a0); break;
+            default: a = matched_cols<5>(yk, sBz, DNV, nu, w, r, ig, az, g0, zlo, a0); break;
+          }
+          lacc[z] = a;
+        }
+      } else if (nu <= 4) {
+        float w[4], r[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          w[t] = rdl(q_w[t], jl);
+          r[t] = rdl(q_r[t], jl);
+        }
+        for (int zz = z0; zz <= z1; zz += 64) {
+          const int z = zz + lane;
+          if (z > z1) continue;
+          const float zlo = (float)z - 0.5f;
+          float a = lacc[z];
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            if (t < nu && w[t] != 0.f)
+              a = fmaf(w[t], matched_rows(yk + t * DNV, sBz, DNV, r[t], az, zlo, d.B0, d.dB),
+                       a);
+          }
+          lacc[z] = a;
+        }
+      } else {
+        // more than four columns: their weights recomputed here, as in the lane stage
+        const int p = vb + k * vs;
+        const ViewP vp = views[p];
+        const int ii = vp.xmarch ? x : y;
+        const int j = vp.xmarch ? y : x;
+        const float fi = (float)ii;
+        const float2* bt = bnd + (size_t)p * (DNU + 1);
+        const float4* ctab = cols + (size_t)p * DNU;
+        const float dir = (float)vp.bdir;
+        const float cl = (vp.bdir > 0) ? (float)j - 0.5f : -((float)j + 0.5f);
+        const float ch = cl + 1.0f;
+        for (int zz = z0; zz <= z1; zz += 64) {
+          const int z = zz + lane;
+          if (z > z1) continue;
+          const float zlo = (float)z - 0.5f;
+          float a = lacc[z];
+          for (int t = 0; t < nu; ++t) {
+            const int u = u0 + t;
+            const float su = dir * fmaf(fi, bt[u].y, bt[u].x);
+            const float su1 = dir * fmaf(fi, bt[u + 1].y, bt[u + 1].x);
+            const float wy = fminf(su1, ch) - fmaxf(su, cl);
+            if (wy > 0.f) {
+              const float4 ct = ctab[u];
+              const float r = fmaf(fi, ct.y, ct.x);
+              const float wyi = wy * __builtin_amdgcn_rcpf((su1 - su) * r);
+              a = fmaf(wyi, matched_rows(yk + t * DNV, sBz, DNV, r, az, zlo, d.B0, d.dB), a);
+            }
+          }
+          lacc[z] = a;
+        }
+      }
+    }
+  }
+  for (int z = lane; z < d.ZN; z += 64) {
+    const float a = lacc[z] * m;
+    if constexpr (UPDATE) {
+      const size_t i = (size_t)col * d.ZN + z;
+      const float nx = fmaf(a, invCol[i], vcol[z]);
+      vcol[z] = nx;
+      if (shadow) shadow[i] = to_bf16_dither(nx, i);
+    } else {
+      vcol[z] = a;
+    }
+  }
+}
+
 // yp[k][u][v] = y[k][u][v] * K(p,u,v) * invdB[v]: the per-ray factors of fp_kernel
 // (y stored as float or BF16, yp FP32).
 template <typename T>
@@ -1012,6 +1322,21 @@ int launch_bp(const Plan& P, const T* prj, float* vol, int vb, int vs, int vc, i
     if constexpr (!std::is_same<T, float>::value) {
       return fail(CTR_ERR_ARG, "the matched back projector reads the FP32 scaled sinogram");
     } else {
+      // windowed kernel while its row table and column sums fit 64 KiB of LDS
+      const size_t lds =
+          ((size_t)((P.d.DNV + 1 + 3) & ~3) + (size_t)kWaves * P.d.ZN) * sizeof(float);
+      if (P.d.ZN <= kWinMaxZ && lds <= 65536) {
+        const dim3 wgrid((P.d.XN * P.d.YN + kWaves - 1) / kWaves);
+        if (accumulate)
+          hipLaunchKernelGGL((bp_matched_win_kernel<true>), wgrid, dim3(kBlock), lds, st, prj,
+                             vol, P.views, P.bnd, P.cols, P.Bz, P.mask, P.d, P.bc, vb, vs, vc,
+                             invCol, shadow);
+        else
+          hipLaunchKernelGGL((bp_matched_win_kernel<false>), wgrid, dim3(kBlock), lds, st, prj,
+                             vol, P.views, P.bnd, P.cols, P.Bz, P.mask, P.d, P.bc, vb, vs, vc,
+                             nullptr, nullptr);
+        return check_launch("bp_matched_win_kernel");
+      }
 #define CTR_BPM(UPD, ZGV, IC, SH)                                                                \
   hipLaunchKernelGGL((bp_matched_kernel<UPD, ZGV>), rgrid, dim3(kBlock), 0, st, prj, vol, P.views, \
                      P.bnd, P.cols, P.Bz, P.mask, P.d, P.bc, vb, vs, vc, IC, SH)

@@ -115,6 +115,8 @@ static int build_tables(const ctr_geometry* g, Plan& P, double half = 0.0) {
   }
   d.B0 = Bz[0];
   d.dB = (float)((bz[DNV] - bz[0]) / dz / DNV);
+  d.dBmin = Bz[1] - Bz[0];
+  for (int v = 1; v < DNV; ++v) d.dBmin = std::min(d.dBmin, Bz[v + 1] - Bz[v]);
 
   // fan angles of the column boundaries seen from the source (rotation invariant)
   const double sid = std::sqrt(x0 * x0 + y0 * y0);

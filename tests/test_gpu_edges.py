@@ -1,5 +1,5 @@
 """GPU edge cases of the projector pair: columns taller than the windowed back
-projector's LDS limit (the z-group kernel then runs), launches over zero views, and a
+projectors' LDS limit (the z-group kernels then run), launches over zero views, and a
 single-view scan, each against the FP64 oracle or the defined empty result."""
 
 import numpy as np
@@ -38,9 +38,11 @@ def test_tall_columns_use_the_group_kernel(oracle, gpu_plan_factory):
     back = plan.back(y, kind="ref")
     ref = oracle.bp(g, y, "sat")
     eb = rel_l2(back, ref)
-    _say(f"  fp rel-L2 {ef:.3e}, bp-ref rel-L2 {eb:.3e}")
-    assert np.any(ref != 0)
-    assert ef <= TOL and eb <= TOL
+    refm = oracle.bp(g, y, "adjoint")
+    em = rel_l2(plan.back(y, kind="matched"), refm)
+    _say(f"  fp rel-L2 {ef:.3e}, bp-ref rel-L2 {eb:.3e}, matched rel-L2 {em:.3e}")
+    assert np.any(ref != 0) and np.any(refm != 0)
+    assert ef <= TOL and eb <= TOL and em <= TOL
 
 
 def test_zero_view_launches(gpu_plan_factory):
@@ -76,3 +78,31 @@ def test_single_view_scan(p, oracle, gpu_plan_factory):
     eb = rel_l2(plan.back(y, kind="ref"), oracle.bp(g1, y, "sat"))
     _say(f"[single view {p}] fp {ef:.3e} bp {eb:.3e}")
     assert ef <= TOL and eb <= TOL
+
+
+@pytest.mark.parametrize("col_width,col_height,DNU,DNV",
+                         [(4.0, 0.8, 15, 15), (0.5, 0.4, 120, 30), (0.5, 3.0, 120, 4)])
+def test_matched_many_cells_per_voxel(col_width, col_height, DNU, DNV, oracle,
+                                      gpu_plan_factory):
+    """Voxels spanning many detector cells: more rows per voxel than the matched
+    adjoint's fixed-count row gather takes (its row walk runs), and more than four
+    columns per voxel (its recomputed-weight path); against the FP64 oracle adjoint and
+    through the adjoint identity."""
+    _say(f"[matched many cells] col {col_width} x {col_height} mm, {DNU} x {DNV}")
+    cfg = G.helical_config(sid=60, sdd=100, DNU=DNU, DNV=DNV, col_width=col_width,
+                           col_height=col_height, views_per_turn=8, turns=2,
+                           feed_mm_per_turn=6.0, N_xy=8, N_z=8, voxel_xy=4.0, voxel_z=3.0)
+    g = G.dd3_geometry(cfg)
+    plan = gpu_plan_factory(g)
+    y = sinogram(g)
+    got = plan.back(y, kind="matched")
+    ref = oracle.bp(g, y, "adjoint")
+    err = rel_l2(got, ref)
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0, 1, g.vol_shape).astype(np.float32)
+    lhs = float(np.dot(plan.forward(x).astype(np.float64).ravel(), y.astype(np.float64).ravel()))
+    rhs = float(np.dot(x.astype(np.float64).ravel(), got.astype(np.float64).ravel()))
+    adj = abs(lhs - rhs) / abs(lhs)
+    _say(f"  matched rel-L2 {err:.3e}, adjoint identity {adj:.3e}")
+    assert np.any(ref != 0)
+    assert err <= TOL and adj <= 1e-6
