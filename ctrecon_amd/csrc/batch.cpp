@@ -355,25 +355,28 @@ int ctr_batch_create(ctr_plan* const* plans, int nscan, float* const* d_x,
     if ((rc = c->tp.alloc(c->frame * maxV * (B->bf ? 2 : 4)))) return rc;
     if (B->bf && (rc = c->xs.alloc(c->NV * 2))) return rc;
     if ((rc = c->partials.alloc((size_t)2 * upv * maxV * sizeof(double)))) return rc;
-    // row / column sums of the scan's subsets (os_sart.m:14-18)
+    // row / column sums of the scan's subsets (os_sart.m:14-18), evaluated as the
+    // per-scan solver does (without arrays of ones while the columns fit the LDS limit)
     Mem ones;
-    const size_t no = std::max(c->NV, c->frame * maxV);
-    if ((rc = ones.alloc(no * sizeof(float)))) return rc;
-    if ((rc = launch_fill(ones.f(), no, 1.0f, B->st))) return rc;
+    const bool ones_bp = P.d.ZN <= kWinMaxZ;
+    if (!ones_bp) {
+      const size_t no = c->NV;
+      if ((rc = ones.alloc(no * sizeof(float)))) return rc;
+      if ((rc = launch_fill(ones.f(), no, 1.0f, B->st))) return rc;
+    }
     c->rowOff.resize(osNum);
     size_t off = 0;
     for (int s = 0; s < osNum; ++s) {
       const int vc = vcount(P, s, osNum);
       c->rowOff[s] = off;
       float* rI = c->rowInv.f() + off * c->frame;
-      if ((rc = launch_fp<float>(P, ones.f(), rI, s, osNum, vc, B->st, nullptr, nullptr,
-                                 nullptr)))
-        return rc;
+      if ((rc = launch_fp_ones(P, rI, s, osNum, vc, B->st))) return rc;
       if ((rc = launch_inv_guard(rI, rI, c->frame * vc, B->st))) return rc;
       float* cI = c->colInv.f() + c->NV * s;
-      if ((rc = launch_bp<float>(P, ones.f(), cI, s, osNum, vc, CTR_BP_REFERENCE, B->st, nullptr,
-                                 0, nullptr)))
-        return rc;
+      rc = ones_bp ? launch_bp_ones(P, cI, s, osNum, vc, B->st)
+                   : launch_bp<float>(P, ones.f(), cI, s, osNum, vc, CTR_BP_REFERENCE, B->st,
+                                      nullptr, 0, nullptr);
+      if (rc) return rc;
       if ((rc = launch_inv_guard(cI, cI, c->NV, B->st))) return rc;
       off += vc;
     }

@@ -185,6 +185,11 @@ int launch_fp(const Plan& P, const TV* vol, TO* out, int vb, int vs, int vc, hip
 template <typename T>
 int launch_bp(const Plan& P, const T* prj, float* vol, int vb, int vs, int vc, int kind,
               hipStream_t st, const float* invCol, int accumulate, uint16_t* shadow);
+// A 1 and B 1 (the OS-SART row and column sums) of the branchless model over the views
+// vb + k vs, k < vc, without reading a volume or sinogram of ones; launch_bp_ones needs
+// ZN <= kWinMaxZ.
+int launch_fp_ones(const Plan& P, float* out, int vb, int vs, int vc, hipStream_t st);
+int launch_bp_ones(const Plan& P, float* vol, int vb, int vs, int vc, hipStream_t st);
 template <typename T>
 int launch_scale_matched(const Plan& P, const T* in, float* out, int vb, int vs, int vc,
                          hipStream_t st);

@@ -94,7 +94,10 @@ __device__ __forceinline__ float fp_cells(const Buf<TV>& V, const CTR_GLOBAL flo
   return s;
 }
 
-template <bool FUSED, bool COLW, typename TV, typename TO>
+// ONES: the projection of a volume of ones (the row sums of OS-SART), read from no
+// volume: the z weights of a footprint sum to its clipped height zh - zl, so a slice
+// adds (zh - zl) * sum_j wy_j mask_j (the same sum in another rounding order).
+template <bool FUSED, bool COLW, typename TV, typename TO, bool ONES = false>
 __device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk, double* slot) {
   const Dims& d = A.d;
   const int p = A.vb + k * A.vs;
@@ -131,6 +134,14 @@ __device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk
       zh = fminf(zh, ZNh);
       const bool zok = zl < zh;
       if (!__any(zok)) return;
+      if constexpr (ONES) {
+        float W = 0.f;
+        for (int ij = 0; ij < nj; ++ij)
+          W = fmaf((ij == 0) ? wj0 : (ij == nj - 1 ? wjl : 1.f), maskf[ii * mS + (j0 + ij) * mC],
+                   W);
+        if (zok) acc = fmaf((zh - zl) * W, inv, acc);
+        return;
+      }
       // first cell k0 = floor(zl + .5) and its upper edge e; with dk = zh - e the overlaps
       // of cells k0, k0 + 1, k0 + 2 are min(zh, e) - zl, clamp(dk, 0, 1), clamp(dk - 1, 0, 1)
       const float fk = floorf(zl + 0.5f);
@@ -266,6 +277,15 @@ __global__ __launch_bounds__(kBlock) void fp_kernel(FpArgs<TV, TO> A) {
   const int lb = xcd_logical_block(hw, gridDim.x * gridDim.y);
   const int k = lb / gridDim.x;
   fp_body<FUSED, COLW, TV, TO>(A, k, lb - k * gridDim.x, A.partials + 2 * hw);
+}
+
+// Forward projection of a volume of ones (A.vol unused).
+template <bool COLW>
+__global__ __launch_bounds__(kBlock) void fp_ones_kernel(FpArgs<float, float> A) {
+  const int hw = blockIdx.y * gridDim.x + blockIdx.x;
+  const int lb = xcd_logical_block(hw, gridDim.x * gridDim.y);
+  const int k = lb / gridDim.x;
+  fp_body<false, COLW, float, float, true>(A, k, lb - k * gridDim.x, nullptr);
 }
 
 // Batched persistent forward projection: each workgroup takes one contiguous range of
@@ -577,7 +597,11 @@ __device__ __forceinline__ void bp_ref_body(const BpArgs<T>& A, int col, int zbl
 // z0 + 64 + lane, ... (z0, z1 from the row shadow, widened by 2 voxels; the per-voxel
 // test decides), and the column's sums live in LDS (lacc: ZN floats of this wave), each
 // updated by one lane per view, views in order.  Per-voxel arithmetic as bp_ref_body.
-template <bool UPDATE, typename T>
+// ONES: the back projection of a sinogram of ones (the column sums of OS-SART), read
+// from no sinogram: the row weights of a shadow sum to its clipped height vhi - vlo and
+// the column weights to Wu = wu0 + (nu - 2) + fh (wu0 alone for one column), so a voxel
+// adds cosVal * Wu * (vhi - vlo) (the same sum in another rounding order).
+template <bool UPDATE, typename T, bool ONES = false>
 __device__ __forceinline__ void bp_win_body(const BpArgs<T>& A, int col, float* lacc) {
   const Dims& d = A.d;
   const BpConsts64& c64 = A.c64;
@@ -626,6 +650,7 @@ __device__ __forceinline__ void bp_win_body(const BpArgs<T>& A, int col, float* 
       const int u0 = __builtin_amdgcn_readlane(q.u0, j), nu = __builtin_amdgcn_readlane(q.nu, j);
       const int z0 = __builtin_amdgcn_readlane(q_z0, j), z1 = __builtin_amdgcn_readlane(q_z1, j);
       const Buf<T> pp(A.prj + (size_t)k * frame, frame);
+      const float Wu = nu == 1 ? wu0 : wu0 + (float)(nu - 2) + fh;
       for (int zz = z0; zz <= z1; zz += 64) {
         const int z = zz + lane;
         const float dZ = fmaf((float)z - d.ocz, d.dz, -vsz);
@@ -634,7 +659,10 @@ __device__ __forceinline__ void bp_win_body(const BpArgs<T>& A, int col, float* 
         const bool vok = z <= z1 && vlo < vhi;
         if (!vok) continue;
         const float cosVal = cws * __builtin_amdgcn_sqrtf(fmaf(dZ, dZ, l2f));
-        lacc[z] = fmaf(cosVal, bp_cell_sum(pp, u0, nu, wu0, fh, vlo, vhi, DNV), lacc[z]);
+        if constexpr (ONES)
+          lacc[z] = fmaf(cosVal, Wu * (vhi - vlo), lacc[z]);
+        else
+          lacc[z] = fmaf(cosVal, bp_cell_sum(pp, u0, nu, wu0, fh, vlo, vhi, DNV), lacc[z]);
       }
     }
   }
@@ -653,6 +681,13 @@ __global__ __launch_bounds__(kBlock) void bp_win_kernel(BpArgs<T> A) {
   extern __shared__ float lds_acc[];
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   bp_win_body<UPDATE, T>(A, blockIdx.x * kWaves + wid, lds_acc + wid * A.d.ZN);
+}
+
+// Back projection of a sinogram of ones (A.prj unused); dynamic LDS as bp_win_kernel.
+__global__ __launch_bounds__(kBlock) void bp_win_ones_kernel(BpArgs<float> A) {
+  extern __shared__ float lds_acc[];
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  bp_win_body<false, float, true>(A, blockIdx.x * kWaves + wid, lds_acc + wid * A.d.ZN);
 }
 
 // Batched windowed reference back projection: flat work list of (scan, block of 4
@@ -1311,6 +1346,25 @@ int launch_fp(const Plan& P, const TV* vol, TO* out, int vb, int vs, int vc, hip
     else hipLaunchKernelGGL((fp_kernel<false, false, TV, TO>), grid, dim3(kBlock), 0, st, A);
   }
   return check_launch("fp_kernel");
+}
+
+int launch_fp_ones(const Plan& P, float* out, int vb, int vs, int vc, hipStream_t st) {
+  if (vc <= 0) return CTR_OK;
+  dim3 grid((P.d.DNU * P.d.DNV + kBlock - 1) / kBlock, vc);
+  const FpArgs<float, float> A =
+      fp_args<float, float>(P, nullptr, out, vb, vs, nullptr, nullptr, nullptr);
+  if (P.d.DNV == 64) hipLaunchKernelGGL((fp_ones_kernel<true>), grid, dim3(kBlock), 0, st, A);
+  else hipLaunchKernelGGL((fp_ones_kernel<false>), grid, dim3(kBlock), 0, st, A);
+  return check_launch("fp_ones_kernel");
+}
+
+int launch_bp_ones(const Plan& P, float* vol, int vb, int vs, int vc, hipStream_t st) {
+  if (P.d.ZN > kWinMaxZ) return fail(CTR_ERR_ARG, "bp of ones: columns over the LDS limit");
+  const BpArgs<float> A = bp_args<float>(P, nullptr, vol, vb, vs, vc, nullptr, nullptr);
+  const dim3 wgrid((P.d.XN * P.d.YN + kWaves - 1) / kWaves);
+  hipLaunchKernelGGL(bp_win_ones_kernel, wgrid, dim3(kBlock),
+                     (size_t)kWaves * P.d.ZN * sizeof(float), st, A);
+  return check_launch("bp_win_ones_kernel");
 }
 
 template <typename T>

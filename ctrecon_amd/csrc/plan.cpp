@@ -715,21 +715,31 @@ int ctr_sart_create(ctr_plan* plan, const float* d_prj, int osNum, int bp_kind, 
   EventPool ev;
   if ((rc = ev.init(2))) return rc;
   CTR_HIP(hipEventRecord(ev.e[0], st));
-  // RowSum{s} = fp(ones), ColSum{s} = bp(ones) (os_sart.m:14-18)
+  // RowSum{s} = fp(ones), ColSum{s} = bp(ones) (os_sart.m:14-18).  The branchless
+  // direct method evaluates them without a volume / sinogram of ones (launch_fp_ones,
+  // launch_bp_ones); the other methods, the matched adjoint and columns over the LDS
+  // limit project an array of ones.
+  const bool ones_fp = P->method == CTR_METHOD_DIRECT;
+  const bool ones_bp = ones_fp && bp_kind == CTR_BP_REFERENCE && d.ZN <= kWinMaxZ;
   DBuf ones;
-  if ((rc = ones.alloc(std::max(NV, frame * S->maxV) * sizeof(float)))) return rc;
-  if ((rc = launch_fill(ones.f(), std::max(NV, frame * S->maxV), 1.0f, st))) return rc;
+  if (!ones_fp || !ones_bp) {
+    if ((rc = ones.alloc(std::max(NV, frame * S->maxV) * sizeof(float)))) return rc;
+    if ((rc = launch_fill(ones.f(), std::max(NV, frame * S->maxV), 1.0f, st))) return rc;
+  }
   S->rowOff.resize(osNum);
   size_t off = 0;
   for (int s = 0; s < osNum; ++s) {
     const int vc = (d.PN - s + osNum - 1) / osNum;
     S->rowOff[s] = off;
     float* rI = S->rowInv.f() + off * frame;
-    if ((rc = fp_any<float>(*P, ones.f(), rI, s, osNum, vc, st, nullptr, nullptr, nullptr)))
-      return rc;
+    rc = ones_fp ? launch_fp_ones(*P, rI, s, osNum, vc, st)
+                 : fp_any<float>(*P, ones.f(), rI, s, osNum, vc, st, nullptr, nullptr, nullptr);
+    if (rc) return rc;
     if ((rc = launch_inv_guard(rI, rI, frame * vc, st))) return rc;
     float* cI = S->colInv.f() + NV * s;
-    if ((rc = bp_any<float>(*P, ones.f(), cI, s, osNum, vc, bp_kind, st, nullptr, 0))) return rc;
+    rc = ones_bp ? launch_bp_ones(*P, cI, s, osNum, vc, st)
+                 : bp_any<float>(*P, ones.f(), cI, s, osNum, vc, bp_kind, st, nullptr, 0);
+    if (rc) return rc;
     if ((rc = launch_inv_guard(cI, cI, NV, st))) return rc;
     off += vc;
   }
