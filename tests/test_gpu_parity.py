@@ -115,6 +115,27 @@ def test_os_sart_parity(kind, oracle, gpu_plan_factory):
     assert np.allclose(res, res_r, rtol=1e-3)
 
 
+@pytest.mark.parametrize("name", ["ragged", "flat"])
+def test_os_sart_first_sweep_parity(name, oracle, gpu_plan_factory):
+    """One sweep from x = 0: the row and column sums (evaluated on the GPU from the
+    footprints' closed-form sums of ones, mask bytes 0 / 1 / 2 in the ragged case) and
+    the first update against the FP64 oracle, which projects arrays of ones."""
+    from ctrecon_amd.plan import DeviceArray
+
+    _say(f"[os-sart first sweep] case={name}")
+    cfg, g = CASES[name]()
+    plan = gpu_plan_factory(g)
+    b = plan.forward(volume(g))
+    dx = DeviceArray(g.vol_shape)
+    dx.zero()
+    res = plan.os_sart(dx, DeviceArray.from_host(b), 5, 1, "ref")
+    xr, res_r = oracle.os_sart(g, b, osNum=5, iterNum=1, bp_kind="ref")
+    err = rel_l2(dx.get(), xr)
+    _say(f"  iterate rel-L2 vs FP64 oracle = {err:.3e}; residual gpu {res} oracle {res_r}")
+    assert err <= 1e-5
+    assert np.allclose(res, res_r, rtol=1e-5)
+
+
 @pytest.mark.parametrize("kind", ["ref", "matched"])
 def test_conjgrad_parity(kind, oracle, gpu_plan_factory):
     from ctrecon_amd.plan import DeviceArray
