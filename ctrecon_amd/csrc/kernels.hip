@@ -344,6 +344,20 @@ __global__ __launch_bounds__(kBlock) void fp_batch_kernel(const FpBatchItem<T>* 
 }
 
 
+// fp_batch_kernel for the one-workgroup-per-item schedule (grid = total): no item loop.
+template <bool FUSED, bool COLW, typename T>
+__global__ __launch_bounds__(kBlock) void fp_batch1_kernel(const FpBatchItem<T>* __restrict__ D,
+                                                           int nscan, int total) {
+  int i0, step, i1;
+  batch_range(D, nscan, total, 2, i0, step, i1);
+  int s = 0;
+  while (s + 1 < nscan && D[s + 1].begin <= i0) ++s;
+  const FpBatchItem<T>& B = D[s];
+  const int local = i0 - B.begin;
+  const int k = local / B.upv;
+  fp_body<FUSED, COLW, T, T>(B.a, k, local - k * B.upv, B.a.partials + 2 * local);
+}
+
 // ---------------------------------------------------------------------------
 // Reference back projector: one wave per (x, y) column, each lane owning ZG
 // voxels z = z0 + lane + 64 g (g < ZG), views looped inside.
@@ -834,6 +848,22 @@ __global__ __launch_bounds__(kBlock) void bp_batch_win_kernel(const BpBatchItem<
     bp_win_body<true, T>(B.a, (item - B.begin) * kWaves + wid, lds_acc + wid * zmax,
                          lds_acc + kWaves * zmax + wid * kStageViews * kStagePad);
   }
+}
+
+// bp_batch_win_kernel for the one-workgroup-per-item schedule (grid = total): no item
+// loop, so nothing stays live around the column body.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bp_batch_win1_kernel(
+    const BpBatchItem<T>* __restrict__ D, int nscan, int total, int zmax) {
+  extern __shared__ float lds_acc[];
+  int i0, step, i1;
+  batch_range(D, nscan, total, 2, i0, step, i1);
+  const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  int s = 0;
+  while (s + 1 < nscan && D[s + 1].begin <= i0) ++s;
+  const BpBatchItem<T>& B = D[s];
+  bp_win_body<true, T>(B.a, (i0 - B.begin) * kWaves + wid, lds_acc + wid * zmax,
+                       lds_acc + kWaves * zmax + wid * kStageViews * kStagePad);
 }
 
 // Batched persistent reference back projection: flat work list of (scan, z block,
@@ -1580,7 +1610,13 @@ int launch_fp_batch(const FpBatchItem<T>* items, int nscan, int total, bool colw
   if (total <= 0) return CTR_OK;
   const int sched = grid <= 0 ? 2 : (strided ? 1 : 0);
   grid = grid <= 0 ? total : std::max(1, std::min(grid, total));
-  if (colw)
+  if (sched == 2 && colw)
+    hipLaunchKernelGGL((fp_batch1_kernel<true, true, T>), dim3(grid), dim3(kBlock), 0, st,
+                       items, nscan, total);
+  else if (sched == 2)
+    hipLaunchKernelGGL((fp_batch1_kernel<true, false, T>), dim3(grid), dim3(kBlock), 0, st,
+                       items, nscan, total);
+  else if (colw)
     hipLaunchKernelGGL((fp_batch_kernel<true, true, T>), dim3(grid), dim3(kBlock), 0, st, items,
                        nscan, total, sched);
   else
@@ -1617,9 +1653,13 @@ int launch_bp_batch_win(const BpBatchItem<T>* items, int nscan, int total, int z
   const int sched = grid <= 0 ? 2 : (strided ? 1 : 0);
   grid = grid <= 0 ? total : std::max(1, std::min(grid, total));
   const size_t stg = sizeof(T) == 2 ? kStageViews * kStagePad : 0;  // staged rows: BF16
-  hipLaunchKernelGGL((bp_batch_win_kernel<T>), dim3(grid), dim3(kBlock),
-                     (size_t)kWaves * (zmax + stg) * sizeof(float), st, items, nscan, total,
-                     sched, zmax);
+  const size_t lds = (size_t)kWaves * (zmax + stg) * sizeof(float);
+  if (sched == 2)
+    hipLaunchKernelGGL((bp_batch_win1_kernel<T>), dim3(grid), dim3(kBlock), lds, st, items,
+                       nscan, total, zmax);
+  else
+    hipLaunchKernelGGL((bp_batch_win_kernel<T>), dim3(grid), dim3(kBlock), lds, st, items,
+                       nscan, total, sched, zmax);
   return check_launch("bp_batch_win_kernel");
 }
 
