@@ -813,8 +813,10 @@ __global__ __launch_bounds__(kBlock) void bp_ref_kernel(BpArgs<T> A) {
 
 // Windowed reference back projection; dynamic LDS: kWaves * ZN floats of column sums,
 // then (BF16 sinograms) kWaves * kStageViews * kStagePad floats of staged rows.
+// At most 72 VGPRs (7 waves per SIMD; no scratch): the staged BF16 update otherwise
+// takes 77 and runs at 6 waves per SIMD.
 template <bool UPDATE, typename T>
-__global__ __launch_bounds__(kBlock) void bp_win_kernel(BpArgs<T> A) {
+__global__ __launch_bounds__(kBlock, 7) void bp_win_kernel(BpArgs<T> A) {
   extern __shared__ float lds_acc[];
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   bp_win_body<UPDATE, T>(A, blockIdx.x * kWaves + wid, lds_acc + wid * A.d.ZN,
