@@ -154,6 +154,11 @@ __global__ __launch_bounds__(kBlock) void fp_sat_kernel(
       const double tl = fmaf(fi, bl.y, bl.x) + 1.0, tr = fmaf(fi, br.y, br.x) + 1.0;
       const float r = fmaf(fi, ct.y, ct.x);
       const double tD = fmaf(r, Bv, vp.az) + 1.0, tU = fmaf(r, Bv1, vp.az) + 1.0;
+      // a footprint wholly outside the volume in z or in-plane clamps the samples of a
+      // pair to the same texels: the slice adds exactly 0 and is skipped
+      if (fmax(tU, tD) - 0.5 <= -1.0 || fmin(tU, tD) - 0.5 >= (double)W ||
+          fmax(tl, tr) - 0.5 <= -1.0 || fmin(tl, tr) - 0.5 >= (double)H)
+        continue;
       const int base = ii * H * W;
       // grouped as two differences so that samples clamped to the same texels
       // cancel exactly (a footprint outside the volume gives exactly 0)
@@ -233,6 +238,9 @@ __global__ __launch_bounds__(kBlock) void bp_sat_kernel(
     const double rl = rsqrt(l2);
     const double iU = (dZ + c.hz) * c.S * rl + c.cV + 1.0;
     const double iD = (dZ - c.hz) * c.S * rl + c.cV + 1.0;
+    // a row shadow wholly below or above the detector clamps all four samples of a
+    // column pair to the same texels: the view adds exactly 0 and is skipped
+    if (fmax(iU, iD) - 0.5 <= -1.0 || fmin(iU, iD) - 0.5 >= (double)W) continue;
     const double alpha = asin((s64.y * Px - s64.x * Py) * c64.inv_sid * rl);
     const double w = vp.wx ? dX : dY;
     const double dA = w * (rl * rl) * (c64.dx * 0.5);
