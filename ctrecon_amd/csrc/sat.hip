@@ -111,12 +111,17 @@ __device__ __forceinline__ double sat_sample(const BufF32& Hi, const BufF32& Lo,
   const double al = a - fa, be = b - fb;
   const int i0 = min(max(i, 0), W - 1), i1 = min(max(i + 1, 0), W - 1);
   const int j0 = min(max(j, 0), H - 1), j1 = min(max(j + 1, 0), H - 1);
-  const int o00 = base + j0 * W + i0, o01 = base + j0 * W + i1;
-  const int o10 = base + j1 * W + i0, o11 = base + j1 * W + i1;
-  const double v00 = (double)Hi[o00] + (double)Lo[o00];
-  const double v01 = (double)Hi[o01] + (double)Lo[o01];
-  const double v10 = (double)Hi[o10] + (double)Lo[o10];
-  const double v11 = (double)Hi[o11] + (double)Lo[o11];
+  // texels i0, i1 of a row are among the adjacent pair q, q + 1 (q = i inside the row,
+  // the end pair when clamped): one two-element load per row and array instead of two
+  const int q = min(max(i, 0), W - 2);
+  const bool s0 = i0 == q, s1 = i1 == q;
+  const int r0 = base + j0 * W + q, r1 = base + j1 * W + q;
+  const float h0x = Hi[r0], h0y = Hi[r0 + 1], l0x = Lo[r0], l0y = Lo[r0 + 1];
+  const float h1x = Hi[r1], h1y = Hi[r1 + 1], l1x = Lo[r1], l1y = Lo[r1 + 1];
+  const double v00 = (double)(s0 ? h0x : h0y) + (double)(s0 ? l0x : l0y);
+  const double v01 = (double)(s1 ? h0x : h0y) + (double)(s1 ? l0x : l0y);
+  const double v10 = (double)(s0 ? h1x : h1y) + (double)(s0 ? l1x : l1y);
+  const double v11 = (double)(s1 ? h1x : h1y) + (double)(s1 ? l1x : l1y);
   return (1.0 - be) * ((1.0 - al) * v00 + al * v01) + be * ((1.0 - al) * v10 + al * v11);
 }
 
