@@ -4,9 +4,10 @@
 //
 // This is the reference's own formulation (DD3_GPU.cu:33-50, 101-197 volume SATs,
 // 1241-1323 sinogram SATs, 844-990 forward, 1397-1475 back), with two changes:
-//  * prefix sums are double-float (hi + lo FP32 pairs accumulated with TwoSum), so
-//    the difference of four nearby corners keeps its low-order bits; whole-plane
-//    FP32 integral images lose ~eps * 1e5 per corner at clinical sizes;
+//  * prefix sums are double-float (hi + lo FP32 pairs accumulated with TwoSum, each
+//    texel stored as the pair (hi, lo)), so the difference of four nearby corners
+//    keeps its low-order bits; whole-plane FP32 integral images lose ~eps * 1e5 per
+//    corner at clinical sizes;
 //  * texture fetches are software bilinear samples with exact weights and clamp
 //    addressing (sample at t - 0.5, clamp each texel index), evaluated in FP64.
 // The direct-footprint kernels in kernels.hip compute the same operator faster;
@@ -55,8 +56,7 @@ struct SinoRows {  // slices k (views of the launch), rows u', columns v'
 // One wave per (slice, row); rows 0 (the border) are written as zeros.
 template <typename Src>
 __global__ __launch_bounds__(kBlock) void sat_rows_kernel(Src src, int NSl, int R, int W,
-                                                          float* __restrict__ hi,
-                                                          float* __restrict__ lo) {
+                                                          float2* __restrict__ hl) {
   const int lane = threadIdx.x & 63;
   const int row = blockIdx.x * kWaves + (threadIdx.x >> 6);
   if (row >= NSl * (R + 1)) return;
@@ -73,10 +73,7 @@ __global__ __launch_bounds__(kBlock) void sat_rows_kernel(Src src, int NSl, int 
       if (lane >= off) v = df_add(o, v);
     }
     v = df_add(carry, v);
-    if (c < W) {
-      hi[base + c] = v.hi;
-      lo[base + c] = v.lo;
-    }
+    if (c < W) hl[base + c] = make_float2(v.hi, v.lo);
     carry.hi = __shfl(v.hi, 63, 64);
     carry.lo = __shfl(v.lo, 63, 64);
   }
@@ -84,26 +81,26 @@ __global__ __launch_bounds__(kBlock) void sat_rows_kernel(Src src, int NSl, int 
 
 // Pass 2: inclusive prefix along the rows of each slice (stride W); one lane per
 // (slice, column), lanes across columns so every access is coalesced.
-__global__ __launch_bounds__(kBlock) void sat_cols_kernel(int R, int W, float* __restrict__ hi,
-                                                          float* __restrict__ lo) {
+__global__ __launch_bounds__(kBlock) void sat_cols_kernel(int R, int W, float2* __restrict__ hl) {
   const int c = blockIdx.x * kBlock + threadIdx.x;
   const int s = blockIdx.y;
   if (c >= W) return;
   size_t o = ((size_t)s * (R + 1)) * W + c;
-  DF acc{hi[o], lo[o]};
+  const float2 v0 = hl[o];
+  DF acc{v0.x, v0.y};
   for (int r = 1; r <= R; ++r) {
     o += W;
-    acc = df_add(acc, DF{hi[o], lo[o]});
-    hi[o] = acc.hi;
-    lo[o] = acc.lo;
+    const float2 v = hl[o];
+    acc = df_add(acc, DF{v.x, v.y});
+    hl[o] = make_float2(acc.hi, acc.lo);
   }
 }
 
 // Clamped bilinear sample of the double-float table plane at unnormalised texture
 // coordinates (a along the W axis, b along the H axis), as cudaFilterModeLinear +
 // cudaAddressModeClamp define it, with exact weights.
-__device__ __forceinline__ double sat_sample(const BufF32& Hi, const BufF32& Lo, int base, int W,
-                                             int H, double a, double b) {
+__device__ __forceinline__ double sat_sample(const BufF32& HL, int base, int W, int H, double a,
+                                             double b) {
   a = fmin(fmax(a - 0.5, -1.0), (double)W);
   b = fmin(fmax(b - 0.5, -1.0), (double)H);
   const double fa = floor(a), fb = floor(b);
@@ -112,12 +109,13 @@ __device__ __forceinline__ double sat_sample(const BufF32& Hi, const BufF32& Lo,
   const int i0 = min(max(i, 0), W - 1), i1 = min(max(i + 1, 0), W - 1);
   const int j0 = min(max(j, 0), H - 1), j1 = min(max(j + 1, 0), H - 1);
   // texels i0, i1 of a row are among the adjacent pair q, q + 1 (q = i inside the row,
-  // the end pair when clamped): one two-element load per row and array instead of two
+  // the end pair when clamped); texel t is (hi, lo) at floats 2t, 2t + 1, so a row's
+  // pair is one run of four floats
   const int q = min(max(i, 0), W - 2);
   const bool s0 = i0 == q, s1 = i1 == q;
-  const int r0 = base + j0 * W + q, r1 = base + j1 * W + q;
-  const float h0x = Hi[r0], h0y = Hi[r0 + 1], l0x = Lo[r0], l0y = Lo[r0 + 1];
-  const float h1x = Hi[r1], h1y = Hi[r1 + 1], l1x = Lo[r1], l1y = Lo[r1 + 1];
+  const int r0 = 2 * (base + j0 * W + q), r1 = 2 * (base + j1 * W + q);
+  const float h0x = HL[r0], l0x = HL[r0 + 1], h0y = HL[r0 + 2], l0y = HL[r0 + 3];
+  const float h1x = HL[r1], l1x = HL[r1 + 1], h1y = HL[r1 + 2], l1y = HL[r1 + 3];
   const double v00 = (double)(s0 ? h0x : h0y) + (double)(s0 ? l0x : l0y);
   const double v01 = (double)(s1 ? h0x : h0y) + (double)(s1 ? l0x : l0y);
   const double v10 = (double)(s0 ? h1x : h1y) + (double)(s0 ? l1x : l1y);
@@ -130,9 +128,7 @@ __device__ __forceinline__ double sat_sample(const BufF32& Hi, const BufF32& Lo,
 // type of out (float, or uint16_t for BF16).
 template <bool FUSED, typename TO>
 __global__ __launch_bounds__(kBlock) void fp_sat_kernel(
-    const float* __restrict__ zyx_hi, const float* __restrict__ zyx_lo,
-    const float* __restrict__ zxy_hi, const float* __restrict__ zxy_lo,
-    TO* __restrict__ out, const ViewP* __restrict__ views, const float2* __restrict__ bnd,
+    const float2* __restrict__ zyx, const float2* __restrict__ zxy, TO* __restrict__ out, const ViewP* __restrict__ views, const float2* __restrict__ bnd,
     const float4* __restrict__ cols, const float* __restrict__ Bz,
     const float* __restrict__ h2, Dims d, int vb, int vs, const float* __restrict__ b,
     const float* __restrict__ invRow, double* __restrict__ partials) {
@@ -152,7 +148,7 @@ __global__ __launch_bounds__(kBlock) void fp_sat_kernel(
     const int NS = vp.xmarch ? d.XN : d.YN;
     const int H = (vp.xmarch ? d.YN : d.XN) + 1, W = d.ZN + 1;
     const uint32_t n = (uint32_t)NS * H * W;
-    const BufF32 Hi(vp.xmarch ? zyx_hi : zxy_hi, n), Lo(vp.xmarch ? zyx_lo : zxy_lo, n);
+    const BufF32 HL(reinterpret_cast<const float*>(vp.xmarch ? zyx : zxy), 2 * n);
     double acc = 0.0;
     for (int ii = 0; ii < NS; ++ii) {
       const float fi = (float)ii;
@@ -167,10 +163,10 @@ __global__ __launch_bounds__(kBlock) void fp_sat_kernel(
       const int base = ii * H * W;
       // grouped as two differences so that samples clamped to the same texels
       // cancel exactly (a footprint outside the volume gives exactly 0)
-      const double diff = (sat_sample(Hi, Lo, base, W, H, tU, tr) -
-                           sat_sample(Hi, Lo, base, W, H, tD, tr)) -
-                          (sat_sample(Hi, Lo, base, W, H, tU, tl) -
-                           sat_sample(Hi, Lo, base, W, H, tD, tl));
+      const double diff = (sat_sample(HL, base, W, H, tU, tr) -
+                           sat_sample(HL, base, W, H, tD, tr)) -
+                          (sat_sample(HL, base, W, H, tU, tl) -
+                           sat_sample(HL, base, W, H, tD, tl));
       acc += diff / ((tr - tl) * (tU - tD));
     }
     const float val = (float)(acc * (double)(ct.w * __builtin_amdgcn_sqrtf(ct.z + h2[v])));
@@ -214,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void fp_sat_kernel(
 // shadow of vol when shadow is not NULL.
 template <bool UPDATE>
 __global__ __launch_bounds__(kBlock) void bp_sat_kernel(
-    const float* __restrict__ shi, const float* __restrict__ slo, float* __restrict__ vol,
+    const float2* __restrict__ sat, float* __restrict__ vol,
     const ViewP* __restrict__ views, const double2* __restrict__ src64,
     const uint8_t* __restrict__ mask, Dims d, BpConsts c, BpConsts64 c64, int vb, int vs,
     int vc, const float* __restrict__ invCol, uint16_t* __restrict__ shadow) {
@@ -232,7 +228,7 @@ __global__ __launch_bounds__(kBlock) void bp_sat_kernel(
   const double Py = ((double)y - c64.ocy) * c64.dx;
   const double Pz = ((double)z - (double)d.ocz) * (double)d.dz;
   const int W = d.DNV + 1, H = d.DNU + 1;
-  const BufF32 Hi(shi, (uint32_t)vc * W * H), Lo(slo, (uint32_t)vc * W * H);
+  const BufF32 HL(reinterpret_cast<const float*>(sat), 2u * (uint32_t)vc * W * H);
   double acc = 0.0;
   for (int k = 0; k < vc; ++k) {
     const int p = vb + k * vs;
@@ -255,8 +251,8 @@ __global__ __launch_bounds__(kBlock) void bp_sat_kernel(
     const int base = k * W * H;
     // grouped as two differences so that a shadow outside the detector gives exactly
     // 0: the OS-SART column-sum guard (os_sart.m:38-39) relies on exact zeros
-    acc += ((sat_sample(Hi, Lo, base, W, H, iU, iR) - sat_sample(Hi, Lo, base, W, H, iD, iR)) -
-            (sat_sample(Hi, Lo, base, W, H, iU, iL) - sat_sample(Hi, Lo, base, W, H, iD, iL))) *
+    acc += ((sat_sample(HL, base, W, H, iU, iR) - sat_sample(HL, base, W, H, iD, iR)) -
+            (sat_sample(HL, base, W, H, iU, iL) - sat_sample(HL, base, W, H, iD, iL))) *
            cosVal;
   }
   if constexpr (UPDATE) {
@@ -275,14 +271,14 @@ int check(const char* what) {
 }
 
 template <typename Src>
-int build_sat(Src src, int NSl, int R, int W, float* hi, float* lo, hipStream_t st) {
+int build_sat(Src src, int NSl, int R, int W, float2* hl, hipStream_t st) {
   const int rows = NSl * (R + 1);
   hipLaunchKernelGGL(sat_rows_kernel<Src>, dim3((rows + kWaves - 1) / kWaves), dim3(kBlock), 0,
-                     st, src, NSl, R, W, hi, lo);
+                     st, src, NSl, R, W, hl);
   int rc = check("sat_rows_kernel");
   if (rc) return rc;
   hipLaunchKernelGGL(sat_cols_kernel, dim3((W + kBlock - 1) / kBlock, NSl), dim3(kBlock), 0, st,
-                     R, W, hi, lo);
+                     R, W, hl);
   return check("sat_cols_kernel");
 }
 
@@ -296,19 +292,17 @@ size_t sat_sino_floats(const Dims& d, int vc) {
   return (size_t)vc * (d.DNU + 1) * (d.DNV + 1);
 }
 
-// Volume integral images into sat (hi planes, then lo planes: ZYX | ZXY each).
+// Volume integral images into sat: ZYX then ZXY texels, each texel (hi, lo).
 template <typename T>
 int launch_volume_sat(const Plan& P, const T* vol, float* sat, hipStream_t st) {
   const Dims& d = P.d;
   const size_t nx = (size_t)d.XN * (d.YN + 1) * (d.ZN + 1);
-  const size_t n = sat_volume_floats(d);
-  float* hi = sat;
-  float* lo = sat + n;
-  int rc = build_sat(VolumeRowsX<T>{vol, P.maskf, d.XN, d.YN, d.ZN}, d.XN, d.YN, d.ZN + 1, hi,
-                     lo, st);
+  float2* hl = reinterpret_cast<float2*>(sat);
+  int rc = build_sat(VolumeRowsX<T>{vol, P.maskf, d.XN, d.YN, d.ZN}, d.XN, d.YN, d.ZN + 1, hl,
+                     st);
   if (rc) return rc;
   return build_sat(VolumeRowsY<T>{vol, P.maskf, d.XN, d.YN, d.ZN}, d.YN, d.XN, d.ZN + 1,
-                   hi + nx, lo + nx, st);
+                   hl + nx, st);
 }
 
 template <typename TO>
@@ -316,17 +310,16 @@ int launch_fp_sat(const Plan& P, const float* sat, TO* out, int vb, int vs, int 
                   hipStream_t st, const float* b, const float* invRow, double* partials) {
   const Dims& d = P.d;
   const size_t nx = (size_t)d.XN * (d.YN + 1) * (d.ZN + 1);
-  const size_t n = sat_volume_floats(d);
-  const float *zyx_hi = sat, *zxy_hi = sat + nx, *zyx_lo = sat + n, *zxy_lo = sat + n + nx;
+  const float2* zyx = reinterpret_cast<const float2*>(sat);
+  const float2* zxy = zyx + nx;
   dim3 grid((d.DNU * d.DNV + kBlock - 1) / kBlock, vc);
   if (b)
-    hipLaunchKernelGGL((fp_sat_kernel<true, TO>), grid, dim3(kBlock), 0, st, zyx_hi, zyx_lo,
-                       zxy_hi, zxy_lo, out, P.views, P.bnd, P.cols, P.Bz, P.h2, d, vb, vs, b,
-                       invRow, partials);
+    hipLaunchKernelGGL((fp_sat_kernel<true, TO>), grid, dim3(kBlock), 0, st, zyx, zxy, out,
+                       P.views, P.bnd, P.cols, P.Bz, P.h2, d, vb, vs, b, invRow, partials);
   else
-    hipLaunchKernelGGL((fp_sat_kernel<false, TO>), grid, dim3(kBlock), 0, st, zyx_hi, zyx_lo,
-                       zxy_hi, zxy_lo, out, P.views, P.bnd, P.cols, P.Bz, P.h2, d, vb, vs,
-                       nullptr, nullptr, nullptr);
+    hipLaunchKernelGGL((fp_sat_kernel<false, TO>), grid, dim3(kBlock), 0, st, zyx, zxy, out,
+                       P.views, P.bnd, P.cols, P.Bz, P.h2, d, vb, vs, nullptr, nullptr,
+                       nullptr);
   return check("fp_sat_kernel");
 }
 
@@ -334,16 +327,16 @@ template <typename T>
 int launch_bp_sat(const Plan& P, const T* prj, float* sat, float* vol, int vb, int vs, int vc,
                   hipStream_t st, const float* invCol, int accumulate, uint16_t* shadow) {
   const Dims& d = P.d;
-  const size_t n = sat_sino_floats(d, vc);
-  int rc = build_sat(SinoRows<T>{prj, d.DNU, d.DNV}, vc, d.DNU, d.DNV + 1, sat, sat + n, st);
+  float2* hl = reinterpret_cast<float2*>(sat);  // texels (hi, lo)
+  int rc = build_sat(SinoRows<T>{prj, d.DNU, d.DNV}, vc, d.DNU, d.DNV + 1, hl, st);
   if (rc) return rc;
   dim3 grid((d.XN * d.YN + kWaves - 1) / kWaves, (d.ZN + 63) / 64);
   if (accumulate)
-    hipLaunchKernelGGL(bp_sat_kernel<true>, grid, dim3(kBlock), 0, st, sat, sat + n, vol,
-                       P.views, P.src64, P.mask, d, P.bc, P.bc64, vb, vs, vc, invCol, shadow);
+    hipLaunchKernelGGL(bp_sat_kernel<true>, grid, dim3(kBlock), 0, st, hl, vol, P.views,
+                       P.src64, P.mask, d, P.bc, P.bc64, vb, vs, vc, invCol, shadow);
   else
-    hipLaunchKernelGGL(bp_sat_kernel<false>, grid, dim3(kBlock), 0, st, sat, sat + n, vol,
-                       P.views, P.src64, P.mask, d, P.bc, P.bc64, vb, vs, vc, nullptr, nullptr);
+    hipLaunchKernelGGL(bp_sat_kernel<false>, grid, dim3(kBlock), 0, st, hl, vol, P.views,
+                       P.src64, P.mask, d, P.bc, P.bc64, vb, vs, vc, nullptr, nullptr);
   return check("bp_sat_kernel");
 }
 
