@@ -649,8 +649,9 @@ __device__ __forceinline__ void bp_ref_body(const BpArgs<T>& A, int col, int zbl
 // helical scan a view's cone covers a window of a few dozen voxels of a column, which
 // the fixed lane -> z groups above cover with one 64-lane pass per group it touches
 // (about half the lanes idle).  Here the lanes take the view's own window, z0 + lane,
-// z0 + 64 + lane, ... (z0, z1 from the row shadow, widened by 2 voxels; the per-voxel
-// test decides), and the column's sums live in LDS (lacc: ZN floats of this wave), each
+// z0 + 64 + lane, ... (z0, z1: the voxels of the row shadow's open interval, with a
+// 1e-2-voxel margin; the per-voxel test decides), and the column's sums live in LDS
+// (lacc: ZN floats of this wave), each
 // updated by one lane per view, views in order.  Per-voxel arithmetic as bp_ref_body.
 // ONES: the back projection of a sinogram of ones (the column sums of OS-SART), read
 // from no sinogram: the row weights of a shadow sum to its clipped height vhi - vlo and
