@@ -408,7 +408,7 @@ def main():
         per_launch_bytes = float(bytes_per_view.sum()) / args.subsets
         views_per_launch = g.PN / args.subsets
         # the reference back projector runs windowed for columns up to 2048 voxels
-        dominant = "bp_win_kernel" if args.bp == "ref" else "bp_matched_kernel"
+        dominant = "bp_win_kernel" if args.bp == "ref" else "bp_matched_win_kernel"
         achieved = per_launch_bytes / (bp_avg / 1e3)
         traffic = None
         tpath = os.path.join(ROOT, "profiles", "traffic.json")
@@ -423,6 +423,10 @@ def main():
             "frac": achieved / HBM_PEAK,
             "traffic": traffic,
             "algorithmic_bytes_per_launch": per_launch_bytes,
+            # compulsory I/O of one update launch: the subset's residual sinogram in, the
+            # iterate and its column sums in, the iterate out
+            "compulsory_bytes_per_launch": (4.0 * frame * views_per_launch
+                                            + 3 * 4.0 * g.XN * g.YN * g.ZN),
             "avg_launch_ms": bp_avg,
         }
         kernels = {"fp_fused": fp_avg, "bp_update": bp_avg}
