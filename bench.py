@@ -14,17 +14,21 @@ target, with the row / column-sum set-up inside the timed region:
   * learned + BF16: BF16 projector operands (FP32 master iterate), two sweeps, the
     warm-start network's predicted update, then plain OS-SART sweeps (with the
     fallback to plain OS-SART when the residual does not drop).
-``value`` is the learned + BF16 time when trained weights are present
-(ctrecon_amd/weights/warm_start.npz; the later-stage networks warm_start_stage2.npz and
-warm_start_stage3.npz act, when present, --after2 / --after3 sweeps after the previous
-network's update), otherwise the plain FP32 time.
+``value`` is the learned + BF16 time to the target (ctrecon_amd/weights/warm_start.npz;
+the later-stage network warm_start_stage2.npz acts, when present, --after2 sweeps after
+the first network's update).  Without trained weights ``value`` is null and the plain
+time is reported alone.
+
+Steps: --warmup W untimed plain FP32 sweeps, then the plain FP32 solve from x_0 = 0 whose
+first K = --steps sweeps are the timed steps (``steps`` = K, ``ms_per_step`` = their mean
+GPU time; if the target is met in fewer sweeps, the solve continues to K sweeps for these
+statistics, outside the time to target).  Both forms always run to the target (at most
+--max-steps sweeps).
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 
     python bench.py [--steps K] [--warmup W] [--target 1e-3] [--form both|plain|learned] [--no-cpu]
-
---steps 0 (default) runs sweeps until the target is met (at most --max-steps);
---steps K runs exactly K sweeps of the plain form (profiling runs).
+    python bench.py --sweeps-only --steps K    # exactly K plain sweeps (profiling runs)
 """
 
 from __future__ import annotations
@@ -50,6 +54,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (as lscpu prints it), from /proc/cpuinfo."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(g, vol, n_views, target_sweeps, setup_sweeps=1.0):
     """The oracle's FP32 restatement of the reference path (integral images + bilinear
     samples, OpenMP over the host cores) on a bounded sample of the same scan."""
@@ -66,16 +82,73 @@ def cpu_baseline(g, vol, n_views, target_sweeps, setup_sweeps=1.0):
     out = {
         "unit": "s",
         "cores": O.num_threads(),
+        "cpu_model": cpu_model(),
+        "logical_cpus": os.cpu_count(),
         "kind": "port",
         "projections_per_s": 1.0 / per_view,
         "fp_s": t1 - t0,
         "bp_s": t2 - t1,
         "sample": (f"{n_views} views spread over the 3936-view scan, FP32 forward + back "
-                   f"projection of the full 512x512x256 volume ({t2 - t0:.1f} s); value = "
-                   f"(sweeps to target + 1 set-up sweep) x 3936 views x measured s/view"),
+                   f"projection of the full 512x512x256 volume ({t2 - t0:.1f} s of CPU work); "
+                   f"value = (plain sweeps to target + 1 set-up sweep) x 3936 views x "
+                   f"measured s/view, extrapolated"),
+        "target_sweeps": target_sweeps,
     }
     out["value"] = (None if target_sweeps is None else
                     (target_sweeps + setup_sweeps) * g.PN * per_view)
+    return out
+
+
+def headline_record(args, plain, learned, learned_sino, roof, pps, kernels, cpu, have_net,
+                    views):
+    """The one-line JSON record of the clinical workload.  `plain` / `learned` are the legs'
+    result dicts (learned None without trained weights); `plain["timed_ms"]` holds the K
+    timed sweeps."""
+    timed = plain["timed_ms"]
+    t_plain = plain["time_to_target_s"]
+    t_learned = learned["time_to_target_s"] if learned is not None else None
+    both = t_plain is not None and t_learned is not None
+    form = ("learned warm start + BF16 storage (FP32 accumulation)" if learned is not None else
+            "plain FP32" + ("" if have_net else " (no trained network weights found)"))
+    out = {
+        "metric": METRIC,
+        "value": t_learned,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": len(timed),
+        "warmup": args.warmup,
+        "ms_per_step": float(np.mean(timed)) if len(timed) else None,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 accumulation, bf16 storage" if learned is not None else "f32",
+        "data": "synthetic: seeded 40-ellipsoid phantom (seed 1) + uniform texture; "
+                "b = A x_true (noise-free FP32)",
+        "config": {
+            "workload": "512x512x256 helical OS-SART to rel. residual "
+                        f"{args.target:g}, " + ("reference back projector" if args.bp == "ref"
+                                                 else "matched adjoint"),
+            "form": form,
+            "step": "one plain FP32 OS-SART sweep (20 subsets, 3936 views forward- and "
+                    "back-projected)",
+            "volume": [512, 512, 256], "voxel_mm": 0.7, "detector": [888, 64],
+            "views": int(views), "views_per_turn": 984, "pitch": 0.9,
+            "subsets": args.subsets, "target_rel_residual": args.target,
+        },
+        "roofline": roof,
+        "sweeps_to_target": learned["sweeps"] if learned is not None else plain[
+            "sweeps_to_target"],
+        "plain_fp32": {k: v for k, v in plain.items() if k != "timed_ms"},
+        "learned_bf16": learned,
+        "learned_bf16_sinogram_only": learned_sino,
+        "speedup_vs_plain": t_plain / t_learned if both else None,
+        "rmse_ratio_vs_plain": (learned["rmse_vs_truth_in_mask"] / plain["rmse_vs_truth_in_mask"]
+                                if learned is not None else None),
+        "projections_per_s": pps,
+        "kernel_avg_ms": kernels,
+    }
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
     return out
 
 
@@ -223,26 +296,25 @@ def projectors_workload(args):
     print(json.dumps(out))
 
 
-def main():
+def parse_args(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=0)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20,
+                    help="timed plain FP32 sweeps for the per-sweep statistics (minimum)")
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--target", type=float, default=1e-3)
     ap.add_argument("--max-steps", type=int, default=400)
     ap.add_argument("--subsets", type=int, default=20)
     ap.add_argument("--bp", default="ref", choices=["ref", "matched"])
-    ap.add_argument("--cpu-views", type=int, default=24)
+    ap.add_argument("--cpu-views", type=int, default=64)
+    ap.add_argument("--sweeps-only", action="store_true",
+                    help="profiling: exactly --steps plain FP32 sweeps, no target, no other leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--form", default="both", choices=["both", "plain", "learned"])
     ap.add_argument("--weights2", default=os.path.join(ROOT, "ctrecon_amd", "weights",
                                                        "warm_start_stage2.npz"),
                     help="second-stage network (used when the file exists)")
     ap.add_argument("--after2", type=int, default=3)
-    ap.add_argument("--weights3", default=os.path.join(ROOT, "ctrecon_amd", "weights",
-                                                       "warm_start_stage3.npz"),
-                    help="third-stage network (used when it and the second stage exist)")
-    ap.add_argument("--after3", type=int, default=2)
     ap.add_argument("--weights", default=os.path.join(ROOT, "ctrecon_amd", "weights",
                                                       "warm_start.npz"))
     ap.add_argument("--workload", default="clinical",
@@ -251,7 +323,11 @@ def main():
                          "projectors: every projector model on one clinical subset")
     ap.add_argument("--calib-copy", action="store_true",
                     help="only run the plain copy kernel (byte-counter calibration)")
-    args = ap.parse_args()
+    return ap.parse_args(argv)
+
+
+def main():
+    args = parse_args()
     if args.gpus != 1:
         log("bench.py measures one GPU: this build has no multi-GPU path (one GPU holds the "
             "whole volume and sinogram); run with --gpus 1.")
@@ -260,7 +336,7 @@ def main():
     from ctrecon_amd import geometry as G
     from ctrecon_amd import phantom as P
     from ctrecon_amd._lib import lib
-    from ctrecon_amd.plan import DeviceArray, Plan, device_count
+    from ctrecon_amd.plan import DeviceArray, Plan, Sart, device_count
 
     if device_count() < 1:
         log("no GPU visible")
@@ -304,39 +380,64 @@ def main():
     if args.form == "learned" and not have_net:
         log(f"no network weights at {args.weights}")
         sys.exit(1)
-    run_learned = args.form in ("both", "learned") and have_net and args.steps == 0
-    run_plain = args.form in ("both", "plain") or not run_learned
-    max_iter = args.steps if args.steps > 0 else args.max_steps
-    stop = 0.0 if args.steps > 0 else args.target
+    K = max(args.steps, 1)
 
-    plain = st = None
-    if run_plain:
-        if args.warmup > 0:
-            d_x.zero()
-            plan.os_sart_run(d_x, d_b, args.subsets, args.warmup, args.bp)
-            log(f"warmup: {args.warmup} sweep(s) done")
+    if args.sweeps_only:
+        # profiling: exactly K plain FP32 sweeps from zero, nothing else
         d_x.zero()
         plan.synchronize()
-        t0 = time.perf_counter()
-        res, sweep_ms, st = plan.os_sart_run(d_x, d_b, args.subsets, max_iter, args.bp, stop)
+        res, sweep_ms, st = plan.os_sart_run(d_x, d_b, args.subsets, K, args.bp)
         plan.synchronize()
-        wall = time.perf_counter() - t0
-        cum = st["setup_ms"] + np.cumsum(sweep_ms)
-        hit = np.nonzero(res <= args.target)[0]
-        plain = {
-            "time_to_target_s": float(cum[hit[0]]) / 1e3 if hit.size else None,
-            "wall_s": wall,
-            "sweeps": len(res),
-            "sweeps_to_target": int(hit[0]) + 1 if hit.size else None,
-            "setup_ms": st["setup_ms"],
-            "ms_per_sweep": float(np.mean(sweep_ms)) if len(res) else None,
-            "residual_sweep_final": float(res[-1]) if len(res) else None,
-            "residual_true_final": plan.residual(d_x, d_b),
-            "rmse_vs_truth_in_mask": rmse_of(d_x),
-        }
-        log(f"plain FP32: {len(res)} sweeps, wall {wall:.2f} s, residual {res[-1]:.3e}, "
-            f"time to target {plain['time_to_target_s']}")
+        print(json.dumps({"sweeps": len(res), "ms_per_sweep": float(np.mean(sweep_ms)),
+                          "residual_final": float(res[-1]), "setup_ms": st["setup_ms"],
+                          "fp_avg_ms": st["fp_ms"] / max(st["fp_launches"], 1),
+                          "bp_avg_ms": st["bp_ms"] / max(st["bp_launches"], 1)}))
+        return
 
+    run_learned = args.form in ("both", "learned") and have_net
+    max_sweeps = max(args.max_steps, K)
+
+    # ---- plain FP32: W untimed warm-up sweeps (plain launches, each one timed with HIP
+    # events: the kernel averages of the roofline), then the solve from zero to the target
+    # (sweeps replayed from a captured graph); its first K sweeps are the timed steps
+    st = None
+    if args.warmup > 0:
+        d_x.zero()
+        _, _, st = plan.os_sart_run(d_x, d_b, args.subsets, args.warmup, args.bp)
+        plan.synchronize()
+        log(f"warmup: {args.warmup} sweep(s) done")
+    d_x.zero()
+    plan.synchronize()
+    t0 = time.perf_counter()
+    res, sweep_ms, _ = plan.os_sart_run(d_x, d_b, args.subsets, max_sweeps, args.bp,
+                                        args.target, stats=False)
+    plan.synchronize()
+    wall = time.perf_counter() - t0
+    hit = np.nonzero(res <= args.target)[0]
+    plain = {
+        "time_to_target_s": wall if hit.size else None,
+        "wall_s": wall,
+        "sweeps": len(res),
+        "sweeps_to_target": int(hit[0]) + 1 if hit.size else None,
+        "ms_per_sweep": float(np.mean(sweep_ms)),
+        "gpu_sweeps_time_to_target_s": (float(np.sum(sweep_ms[:hit[0] + 1])) / 1e3
+                                        if hit.size else None),
+        "residual_sweep_final": float(res[-1]),
+        "residual_true_final": plan.residual(d_x, d_b),
+        "rmse_vs_truth_in_mask": rmse_of(d_x),
+        "timed_ms": [float(v) for v in sweep_ms[:K]],
+    }
+    log(f"plain FP32: {len(res)} sweeps, wall {wall:.2f} s, residual {res[-1]:.3e}, "
+        f"time to target {plain['time_to_target_s']}")
+    if len(res) < K:  # fewer sweeps than the timed steps: continue for the statistics only
+        S = Sart(plan, d_b, args.subsets, args.bp)
+        _, more_ms, _ = S.sweeps(d_x, K - len(res))
+        S.close()
+        plain["timed_ms"] += [float(v) for v in more_ms]
+    log(f"timed steps: {len(plain['timed_ms'])} plain FP32 sweeps, "
+        f"{np.mean(plain['timed_ms']):.1f} ms each")
+
+    # ---- the learned warm start with BF16 operands, from zero to the target
     learned = learned_sino = None
     if run_learned:
         from ctrecon_amd.learned import warm_start_os_sart
@@ -346,8 +447,6 @@ def main():
         stages = []
         if os.path.exists(args.weights2):
             stages.append((UNet.from_file(args.weights2), args.after2))
-            if os.path.exists(args.weights3):
-                stages.append((UNet.from_file(args.weights3), args.after3))
 
         def learned_leg(storage):
             plan.set_storage(storage)
@@ -361,8 +460,9 @@ def main():
             d_x.zero()
             plan.synchronize()
             t0 = time.perf_counter()
-            info = warm_start_os_sart(plan, net, d_b, d_x, args.subsets, args.target, max_iter,
-                                      args.bp, stages=stages)
+            info = warm_start_os_sart(plan, net, d_b, d_x, args.subsets, args.target,
+                                      max_sweeps, args.bp, stages=stages)
+            plan.synchronize()
             wall = time.perf_counter() - t0
             plan.set_storage("f32")
             r = np.array(info["residuals"])
@@ -379,8 +479,7 @@ def main():
                 "residual_true_final": plan.residual(d_x, d_b),
                 "rmse_vs_truth_in_mask": rmse_of(d_x),
                 "weights": os.path.relpath(args.weights, ROOT),
-                "later_stages": [os.path.relpath(f, ROOT) for f, _ in
-                                 zip([args.weights2, args.weights3], stages)],
+                "later_stages": [os.path.relpath(args.weights2, ROOT)] if stages else [],
             }
             log(f"learned + {storage}: {info['sweeps']} sweeps, wall {wall:.2f} s, "
                 f"residual {r[-1]:.3e}, fallback {info['fallback']}")
@@ -391,92 +490,57 @@ def main():
         learned = learned_leg("bf16")
         learned_sino = learned_leg("bf16_sino")
 
-    main_form = learned if learned is not None else plain
-    t_target = main_form["time_to_target_s"]
-    steps = main_form["sweeps"]
-    sweeps_target = (learned["sweeps"] if learned is not None and t_target is not None
-                     else plain["sweeps_to_target"])
-
-    roof = kernels = pps = None
-    if st is not None:
-        # roofline of the dominant kernel (the back-projection update)
-        fp_avg = st["fp_ms"] / max(st["fp_launches"], 1)
-        bp_avg = st["bp_ms"] / max(st["bp_launches"], 1)
-        cone = plan.cone_voxels()
-        frame = g.DNU * g.DNV
-        bytes_per_view = 4.0 * cone + 4.0 * frame
-        per_launch_bytes = float(bytes_per_view.sum()) / args.subsets
-        views_per_launch = g.PN / args.subsets
-        # the reference back projector runs windowed for columns up to 2048 voxels
-        dominant = "bp_win_kernel" if args.bp == "ref" else "bp_matched_win_kernel"
-        achieved = per_launch_bytes / (bp_avg / 1e3)
-        traffic = None
-        tpath = os.path.join(ROOT, "profiles", "traffic.json")
-        if os.path.exists(tpath):
-            traffic = json.load(open(tpath)).get(dominant)
-        roof = {
-            "kernel": dominant,
-            "bound": "hbm",
-            "achieved": achieved / 1e9,
-            "peak": HBM_PEAK / 1e9,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK,
-            "traffic": traffic,
-            "algorithmic_bytes_per_launch": per_launch_bytes,
-            # compulsory I/O of one update launch: the subset's residual sinogram in, the
-            # iterate and its column sums in, the iterate out
-            "compulsory_bytes_per_launch": (4.0 * frame * views_per_launch
-                                            + 3 * 4.0 * g.XN * g.YN * g.ZN),
-            "avg_launch_ms": bp_avg,
-        }
-        kernels = {"fp_fused": fp_avg, "bp_update": bp_avg}
-        pps = {
-            "os_sart_loop": g.PN / (plain["ms_per_sweep"] / 1e3) if plain["sweeps"] else None,
-            "fp_kernel": views_per_launch / (fp_avg / 1e3),
-            "bp_kernel": views_per_launch / (bp_avg / 1e3),
-        }
-    form = ("learned warm start + BF16 storage (FP32 accumulation)" if learned is not None else
-            "plain FP32" + ("" if have_net else " (no trained network weights found)"))
-    out = {
-        "metric": METRIC,
-        "value": t_target,
-        "unit": "s",
-        "n_gpus": 1,
-        "steps": steps,
-        "warmup": args.warmup,
-        "ms_per_step": (learned["phase_s"]["post_sweeps_s"] * 1e3 / max(learned["sweeps"] - 2, 1)
-                        if learned is not None else plain["ms_per_sweep"]),
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 accumulation, bf16 storage" if learned is not None else "f32",
-        "data": "synthetic: seeded 40-ellipsoid phantom (seed 1) + uniform texture; "
-                "b = A x_true (noise-free FP32)",
-        "config": {
-            "workload": "512x512x256 helical OS-SART to rel. residual "
-                        f"{args.target:g}, " + ("reference back projector" if args.bp == "ref"
-                                                 else "matched adjoint"),
-            "form": form,
-            "volume": [512, 512, 256], "voxel_mm": 0.7, "detector": [888, 64],
-            "views": int(g.PN), "views_per_turn": 984, "pitch": 0.9, "subsets": args.subsets,
-            "target_rel_residual": args.target,
-        },
-        "roofline": roof,
-        "sweeps_to_target": sweeps_target,
-        "plain_fp32": plain,
-        "learned_bf16": learned,
-        "learned_bf16_sinogram_only": learned_sino,
-        "speedup_vs_plain": (plain["wall_s"] / learned["wall_s"]
-                             if plain is not None and learned is not None else None),
-        "rmse_ratio_vs_plain": (learned["rmse_vs_truth_in_mask"] / plain["rmse_vs_truth_in_mask"]
-                                if plain is not None and learned is not None else None),
-        "projections_per_s": pps,
-        "kernel_avg_ms": kernels,
+    # ---- roofline of the dominant kernel (the back-projection update)
+    if st is None:  # no warm-up sweeps: time two sweeps' launches now
+        d_x.zero()
+        _, _, st = plan.os_sart_run(d_x, d_b, args.subsets, 2, args.bp)
+    plain["setup_ms"] = st["setup_ms"]
+    fp_avg = st["fp_ms"] / max(st["fp_launches"], 1)
+    bp_avg = st["bp_ms"] / max(st["bp_launches"], 1)
+    cone = plan.cone_voxels()
+    frame = g.DNU * g.DNV
+    bytes_per_view = 4.0 * cone + 4.0 * frame
+    per_launch_bytes = float(bytes_per_view.sum()) / args.subsets
+    views_per_launch = g.PN / args.subsets
+    dominant = "bp_win_kernel" if args.bp == "ref" else "bp_matched_win_kernel"
+    achieved = per_launch_bytes / (bp_avg / 1e3)
+    counters = {}
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        counters = json.load(open(tpath)).get(dominant + "_counters", {})
+    roof = {
+        "kernel": dominant,
+        # the named roofline of the projectors (no MFMA in them); the counters below say
+        # which unit limits the kernel
+        "bound": "hbm",
+        "achieved": achieved / 1e9,
+        "peak": HBM_PEAK / 1e9,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK,
+        "traffic": counters.get("hbm_bytes_per_launch"),
+        "traffic_source": counters.get("source"),
+        "limiter": counters.get("limiter"),
+        "valu_issue_frac": counters.get("valu_issue_frac"),
+        "algorithmic_bytes_per_launch": per_launch_bytes,
+        # compulsory I/O of one update launch: the subset's residual sinogram in, the
+        # iterate and its column sums in, the iterate out
+        "compulsory_bytes_per_launch": (4.0 * frame * views_per_launch
+                                        + 3 * 4.0 * g.XN * g.YN * g.ZN),
+        "avg_launch_ms": bp_avg,
+        "launches_timed": st["bp_launches"],
     }
+    kernels = {"fp_fused": fp_avg, "bp_update": bp_avg}
+    pps = {
+        "os_sart_loop": g.PN / (plain["ms_per_sweep"] / 1e3),
+        "fp_kernel": views_per_launch / (fp_avg / 1e3),
+        "bp_kernel": views_per_launch / (bp_avg / 1e3),
+    }
+    cpu = None
     if not args.no_cpu:
         log("cpu baseline sample")
-        out["cpu_baseline"] = cpu_baseline(g, x_true, args.cpu_views,
-                                           plain["sweeps_to_target"] if plain else sweeps_target)
+        cpu = cpu_baseline(g, x_true, args.cpu_views, plain["sweeps_to_target"])
+    out = headline_record(args, plain, learned, learned_sino, roof, pps, kernels, cpu,
+                          have_net, g.PN)
     print(json.dumps(out))
 
 

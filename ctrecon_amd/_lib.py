@@ -41,6 +41,7 @@ EXPORTS = [
     "ctr_malloc", "ctr_free", "ctr_memcpy_h2d", "ctr_memcpy_d2h", "ctr_memset",
     "ctr_synchronize", "ctr_device_copy", "ctr_device_sub", "ctr_device_axpy", "ctr_last_error", "ctr_version",
     "ctr_sart_create", "ctr_sart_sweeps", "ctr_sart_setup_ms", "ctr_sart_destroy",
+    "ctr_sart_set_graph", "ctr_sart_graph_info", "ctr_debug_fail_alloc_after",
     "ctr_batch_create", "ctr_batch_sweeps", "ctr_batch_set_graph", "ctr_batch_set_tuning",
     "ctr_batch_get_info", "ctr_batch_destroy",
     "ctr_unet_num_params", "ctr_unet_create", "ctr_unet_destroy", "ctr_unet_set_params",
@@ -82,7 +83,7 @@ class CtrBatchInfo(ctypes.Structure):
         ("bp_zg", ctypes.c_int), ("bp_grid", ctypes.c_int), ("bp_strided", ctypes.c_int),
         ("setup_ms", ctypes.c_double), ("tune_ms", ctypes.c_double),
         ("capture_ms", ctypes.c_double), ("fp_items", ctypes.c_longlong),
-        ("bp_items", ctypes.c_longlong),
+        ("bp_items", ctypes.c_longlong), ("from_cache", ctypes.c_int),
     ]
 
 
@@ -130,6 +131,9 @@ def lib():
         L.ctr_sart_sweeps.argtypes = [p, p, i, d, p, p, ctypes.POINTER(CtrSartStats)]
         L.ctr_sart_setup_ms.argtypes = [p]
         L.ctr_sart_destroy.argtypes = [p]
+        L.ctr_sart_set_graph.argtypes = [p, i]
+        L.ctr_sart_graph_info.argtypes = [p, ctypes.POINTER(i), ctypes.POINTER(d)]
+        L.ctr_debug_fail_alloc_after.argtypes = [ctypes.c_longlong]
         L.ctr_batch_create.argtypes = [p, i, p, p, i, i, ctypes.POINTER(p)]
         L.ctr_batch_sweeps.argtypes = [p, i, d, p, p, ctypes.POINTER(i)]
         L.ctr_batch_set_graph.argtypes = [p, i]

@@ -16,7 +16,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libctrecon_amd.so")
 ARCH = os.environ.get("CTRECON_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["kernels.hip", "sat.hip", "pdd.hip", "plan.cpp", "batch.cpp", "unet.hip", "unet.cpp"]
+# every HIP / C++ source of csrc/ goes into the one library
+SOURCES = sorted(f for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",
@@ -34,16 +35,31 @@ def hipcc():
     return cand if os.path.exists(cand) else "hipcc"
 
 
-def build(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-shared", *srcs, "-o", tmp]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libctrecon_amd.so")
+        raise RuntimeError(f"hipcc failed: {' '.join(cmd)}")
+
+
+def build(verbose: bool = False, jobs: int = 8) -> str:
+    """Compiles every source to an object (in parallel, into _build/) and links the
+    shared library."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    obj_dir = os.path.join(HERE, "_build")
+    os.makedirs(obj_dir, exist_ok=True)
+    cmds, objs = [], []
+    for src in SOURCES:
+        obj = os.path.join(obj_dir, src + ".o")
+        objs.append(obj)
+        cmds.append([hipcc(), *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+    with ThreadPoolExecutor(max_workers=max(1, min(jobs, len(cmds)))) as ex:
+        list(ex.map(lambda c: _run(c, verbose), cmds))
+    tmp = LIB + ".tmp"
+    _run([hipcc(), f"--offload-arch={ARCH}", "-shared", *objs, "-o", tmp], verbose)
     os.replace(tmp, LIB)
     return LIB
 

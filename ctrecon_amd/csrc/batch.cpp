@@ -8,16 +8,24 @@
 // subset step).  A whole sweep (3 * osNum launches) is captured into a HIP graph and
 // replayed; the residual readback between sweeps stays outside the graph.  The
 // persistent grid sizes, the forward work mapping and the back projector's z groups
-// per lane are chosen by timing the candidates on the batch itself (autotune).
+// per lane are chosen by timing the candidates on the batch itself (autotune); the
+// choice is kept in a cache file keyed by the device and the batch's geometry, so a
+// later batch of the same scans skips the timing.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <memory>
+#include <sstream>
 #include <string>
 #include <vector>
+
+#include <sys/stat.h>
 
 #include "ctr_internal.hpp"
 
@@ -31,7 +39,7 @@ struct Mem {
     if (p) (void)hipFree(p);
   }
   int alloc(size_t bytes) {
-    if (hipMalloc(&p, std::max<size_t>(bytes, 1)) != hipSuccess) {
+    if (dev_malloc(&p, bytes) != hipSuccess) {
       p = nullptr;
       return fail(CTR_ERR_NOMEM, "batch: device allocation failed");
     }
@@ -78,6 +86,7 @@ struct ctr_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   bool useGraph = true;
+  bool fromCache = false;
   double setup_ms = 0, tune_ms = 0, capture_ms = 0;
   ~ctr_batch() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -236,6 +245,75 @@ void drop_graph(ctr_batch& B) {
   B.graph = nullptr;
 }
 
+// ---- the tuning cache: one line per batch, "<key>\t<fp_colw fp_grid fp_strided bp_zg
+// bp_grid bp_strided>"; the last line of a key wins.  Path: $CTRECON_TUNE_CACHE, else
+// $XDG_CACHE_HOME/ctrecon_amd/batch_tuning.txt, else $HOME/.cache/ctrecon_amd/...;
+// a cache that cannot be read or written only means tuning again.
+std::string cache_path() {
+  if (const char* p = std::getenv("CTRECON_TUNE_CACHE")) return p;
+  std::string dir;
+  if (const char* x = std::getenv("XDG_CACHE_HOME")) dir = x;
+  else if (const char* h = std::getenv("HOME")) dir = std::string(h) + "/.cache";
+  else return "";
+  (void)::mkdir(dir.c_str(), 0755);  // an existing folder is fine
+  dir += "/ctrecon_amd";
+  (void)::mkdir(dir.c_str(), 0755);
+  return dir + "/batch_tuning.txt";
+}
+
+// The device (architecture, CU count) and every scan's sizes, voxel sizes, source z
+// range and the subset count: what the launch costs depend on.
+std::string cache_key(const ctr_batch& B) {
+  hipDeviceProp_t prop;
+  std::ostringstream k;
+  if (hipGetDeviceProperties(&prop, B.device) == hipSuccess)
+    k << prop.gcnArchName << "/" << prop.multiProcessorCount;
+  k << "|os" << B.osNum << (B.bf ? "|bf16" : "|f32");
+  char buf[256];
+  for (const auto& c : B.scans) {
+    const Plan& P = *c->P;
+    std::snprintf(buf, sizeof buf, "|%dx%dx%d,%dx%d,%d,%.5g,%.5g,%.5g,%.5g", P.d.XN, P.d.YN,
+                  P.d.ZN, P.d.DNU, P.d.DNV, P.d.PN, P.d.dx, P.d.dz, P.hg.sz.front(),
+                  P.hg.sz.back());
+    k << buf;
+  }
+  return k.str();
+}
+
+bool cache_load(ctr_batch& B, const std::string& key) {
+  const std::string path = cache_path();
+  if (path.empty()) return false;
+  std::ifstream f(path);
+  std::string line, found;
+  while (std::getline(f, line)) {
+    const size_t tab = line.find('\t');
+    if (tab != std::string::npos && line.compare(0, tab, key) == 0 && tab == key.size())
+      found = line.substr(tab + 1);
+  }
+  if (found.empty()) return false;
+  int v[6];
+  std::istringstream in(found);
+  for (int& x : v)
+    if (!(in >> x)) return false;
+  if (v[3] < 0 || v[3] > kZgWindow || v[1] < 0 || v[4] < 0) return false;
+  B.colw = v[0] != 0;
+  B.fpGrid = v[1];
+  B.fpStrided = v[2] != 0;
+  B.zg = v[3];
+  B.bpGrid = v[4];
+  B.bpStrided = v[5] != 0;
+  return true;
+}
+
+void cache_store(const ctr_batch& B, const std::string& key) {
+  const std::string path = cache_path();
+  if (path.empty()) return;
+  std::ofstream f(path, std::ios::app);
+  if (f)
+    f << key << '\t' << (B.colw ? 1 : 0) << ' ' << B.fpGrid << ' ' << B.fpStrided << ' ' << B.zg
+      << ' ' << B.bpGrid << ' ' << B.bpStrided << '\n';
+}
+
 int autotune(ctr_batch& B) {
   bool all64 = true;
   int maxZN = 0;
@@ -392,12 +470,19 @@ int ctr_batch_create(ctr_plan* const* plans, int nscan, float* const* d_x,
   (void)hipEventDestroy(e1);
   if ((rc = B->sums.alloc((size_t)nscan * osNum * 2 * sizeof(double)))) return rc;
   if ((rc = B->zeros.alloc(maxNV * sizeof(float)))) return rc;
-  CTR_HIP(hipMemsetAsync(B->zeros.p, 0, maxNV * sizeof(float), B->st));
+  if ((rc = launch_fill(B->zeros.f(), maxNV, 0.0f, B->st))) return rc;
   if (B->bf)
     for (auto& c : B->scans)
       if ((rc = launch_to_bf16_shadow(c->xs.h(), c->x, c->NV, B->st))) return rc;
   const auto t0 = std::chrono::steady_clock::now();
-  if ((rc = autotune(*B))) return rc;
+  const std::string key = cache_key(*B);
+  if (cache_load(*B, key)) {
+    B->fromCache = true;
+    if ((rc = build(*B))) return rc;
+  } else {
+    if ((rc = autotune(*B))) return rc;
+    cache_store(*B, key);
+  }
   CTR_HIP(hipStreamSynchronize(B->st));
   B->tune_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   *out = B.release();
@@ -440,6 +525,7 @@ int ctr_batch_get_info(const ctr_batch* B, ctr_batch_info* info) {
   info->setup_ms = B->setup_ms;
   info->tune_ms = B->tune_ms;
   info->capture_ms = B->capture_ms;
+  info->from_cache = B->fromCache ? 1 : 0;
   long long f = 0, b = 0;
   for (int s = 0; s < B->osNum; ++s) {
     f += B->fpTotal[s];

@@ -163,7 +163,29 @@ struct Plan {
   double* zl_cpre = nullptr;        // z-line back projector: row prefix of one launch's views
   size_t zl_cpre_n = 0;
   size_t partials_n = 0;
+  // the plan's own stream (default priority, synchronises with the null stream): the
+  // OS-SART / CG drivers, the residual and the host-buffer entry points run on it
+  hipStream_t stream = nullptr;
+  // conjugate-gradient vectors kept between calls: r, p, t, the sinogram between the
+  // projectors, the BF16 copy of the operator's input; and the residual's two sums
+  void* cg[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  size_t cg_bytes[5] = {0, 0, 0, 0, 0};
+  double* sums2 = nullptr;
 };
+
+// Every device allocation of the library goes through dev_malloc (hipMalloc, with the
+// allocation-failure injection of ctr_debug_fail_alloc_after).
+hipError_t dev_malloc(void** p, size_t bytes);
+// *p holds at least `need` bytes afterwards (reallocated, contents lost, when smaller).
+int ensure_bytes(void** p, size_t* have, size_t need, const char* what);
+
+// Relative data residual ||A x - b|| / ||b|| over all views (FP64 sums, added in view
+// order), forward-projecting `chunk` views at a time into prj ([chunk][DNU][DNV]) with
+// partials (2 fp_partial_blocks(P, chunk) doubles) and sums (kResidualSums doubles) as
+// scratch, on stream st; synchronous.  Allocates nothing for the direct method.
+constexpr int kResidualSums = 128;
+int residual_run(Plan& P, const float* d_x, const float* d_prj, float* prj, int chunk,
+                 double* partials, double* sums, hipStream_t st, double* out);
 
 // error handling
 void set_error(const std::string& msg);

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -18,6 +20,9 @@ namespace ctr {
 namespace {
 thread_local std::string g_last_error;
 constexpr double kTwoPi = 6.283185307179586;  // utilities.cuh:75
+// allocation-failure injection: the allocation that finds 0 here fails (and disarms it);
+// -1: off
+std::atomic<long long> g_alloc_countdown{-1};
 }  // namespace
 
 const char* last_error_cstr() { return g_last_error.c_str(); }
@@ -27,6 +32,33 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 int fail(int code, const std::string& msg) {
   g_last_error = msg;
   return code;
+}
+
+hipError_t dev_malloc(void** p, size_t bytes) {
+  long long n = g_alloc_countdown.load();
+  while (n >= 0) {
+    if (g_alloc_countdown.compare_exchange_weak(n, n == 0 ? -1 : n - 1)) {
+      if (n == 0) {
+        *p = nullptr;
+        return hipErrorOutOfMemory;
+      }
+      break;
+    }
+  }
+  return hipMalloc(p, std::max<size_t>(bytes, 1));
+}
+
+int ensure_bytes(void** p, size_t* have, size_t need, const char* what) {
+  if (*have >= need && *p) return CTR_OK;
+  (void)hipFree(*p);
+  *p = nullptr;
+  *have = 0;
+  if (dev_malloc(p, need) != hipSuccess) {
+    *p = nullptr;
+    return fail(CTR_ERR_NOMEM, std::string("cannot allocate ") + what);
+  }
+  *have = need;
+  return CTR_OK;
 }
 
 namespace {
@@ -59,7 +91,10 @@ bool finite_array(const float* a, int n) {
 
 template <typename T>
 int upload(T** dst, const std::vector<T>& src) {
-  CTR_HIP(hipMalloc((void**)dst, sizeof(T) * std::max<size_t>(src.size(), 1)));
+  if (dev_malloc((void**)dst, sizeof(T) * std::max<size_t>(src.size(), 1)) != hipSuccess) {
+    *dst = nullptr;
+    return fail(CTR_ERR_NOMEM, "cannot allocate the plan's geometry tables");
+  }
   CTR_HIP(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
   return CTR_OK;
 }
@@ -296,6 +331,9 @@ static void free_plan(Plan* P) {
   (void)hipFree(P->sat_vol);
   (void)hipFree(P->sat_sino);
   (void)hipFree(P->partials);
+  for (void* b : P->cg) (void)hipFree(b);
+  (void)hipFree(P->sums2);
+  if (P->stream) (void)hipStreamDestroy(P->stream);
   delete P;
 }
 
@@ -305,7 +343,7 @@ static int ensure_scratch_prj(Plan& P, size_t views) {
   P.scratch_prj = nullptr;
   P.scratch_prj_views = 0;
   const size_t bytes = views * (size_t)P.d.DNU * P.d.DNV * sizeof(float);
-  if (hipMalloc((void**)&P.scratch_prj, bytes) != hipSuccess)
+  if (dev_malloc((void**)&P.scratch_prj, bytes) != hipSuccess)
     return fail(CTR_ERR_NOMEM, "cannot allocate sinogram scratch");
   P.scratch_prj_views = views;
   return CTR_OK;
@@ -316,7 +354,7 @@ static int ensure_partials(Plan& P, size_t n) {
   (void)hipFree(P.partials);
   P.partials = nullptr;
   P.partials_n = 0;
-  if (hipMalloc((void**)&P.partials, n * sizeof(double)) != hipSuccess)
+  if (dev_malloc((void**)&P.partials, n * sizeof(double)) != hipSuccess)
     return fail(CTR_ERR_NOMEM, "cannot allocate reduction scratch");
   P.partials_n = n;
   return CTR_OK;
@@ -334,7 +372,7 @@ static int ensure_buf(float** p, size_t* have, size_t need, const char* what) {
   (void)hipFree(*p);
   *p = nullptr;
   *have = 0;
-  if (hipMalloc((void**)p, need * sizeof(float)) != hipSuccess)
+  if (dev_malloc((void**)p, need * sizeof(float)) != hipSuccess)
     return fail(CTR_ERR_NOMEM, std::string("cannot allocate ") + what);
   *have = need;
   return CTR_OK;
@@ -380,7 +418,7 @@ static int bp_any(Plan& P, const T* prj, float* vol, int vb, int vs, int vc, int
         (void)hipFree(P.zl_cpre);
         P.zl_cpre = nullptr;
         P.zl_cpre_n = 0;
-        if (hipMalloc((void**)&P.zl_cpre, std::max<size_t>(need, 1) * sizeof(double)) !=
+        if (dev_malloc((void**)&P.zl_cpre, std::max<size_t>(need, 1) * sizeof(double)) !=
             hipSuccess)
           return fail(CTR_ERR_NOMEM, "cannot allocate the z-line row prefix");
         P.zl_cpre_n = need;
@@ -435,7 +473,7 @@ struct DBuf {
     if (p) (void)hipFree(p);
   }
   int alloc(size_t bytes) {
-    if (hipMalloc(&p, std::max<size_t>(bytes, 1)) != hipSuccess) {
+    if (dev_malloc(&p, bytes) != hipSuccess) {
       p = nullptr;
       return fail(CTR_ERR_NOMEM, "device allocation failed");
     }
@@ -445,6 +483,33 @@ struct DBuf {
   double* d() const { return (double*)p; }
   uint16_t* h() const { return (uint16_t*)p; }
 };
+
+int residual_run(Plan& P, const float* d_x, const float* d_prj, float* prj, int chunk,
+                 double* partials, double* sums, hipStream_t st, double* out) {
+  const int PN = P.d.PN;
+  const int nch = (PN + chunk - 1) / chunk;
+  double h[kResidualSums];
+  double rr = 0, bb = 0;
+  int rc, pending = 0;
+  for (int c = 0; c < nch; ++c) {
+    const int v0 = c * chunk, vc = std::min(chunk, PN - v0);
+    if ((rc = fp_any(P, d_x, prj, v0, 1, vc, st, d_prj, nullptr, partials))) return rc;
+    if ((rc = launch_reduce_pairs(partials, fp_partial_blocks(P, vc), sums + 2 * pending, st)))
+      return rc;
+    // the chunks' sums are read back kResidualSums / 2 at a time and added in chunk order
+    if (++pending == kResidualSums / 2 || c == nch - 1) {
+      CTR_HIP(hipMemcpyAsync(h, sums, 2 * pending * sizeof(double), hipMemcpyDeviceToHost, st));
+      CTR_HIP(hipStreamSynchronize(st));
+      for (int i = 0; i < pending; ++i) {
+        rr += h[2 * i];
+        bb += h[2 * i + 1];
+      }
+      pending = 0;
+    }
+  }
+  *out = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
+  return CTR_OK;
+}
 
 }  // namespace ctr
 
@@ -460,8 +525,22 @@ struct ctr_sart {
   bool bf = false;      // BF16 shadow of the iterate read by the forward projector
   bool sino16 = false;  // BF16 residual sinogram between the projectors
   double setup_ms = 0.0;
-  DBuf rowInv, colInv, tp, xs, sums;
+  DBuf rowInv, colInv, tp, xs, sums, parts;
   std::vector<size_t> rowOff;
+  // one sweep captured as a HIP graph for the iterate graph_x (replayed per sweep)
+  bool use_graph = true;
+  float* graph_x = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double capture_ms = 0.0;
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    graph_x = nullptr;
+  }
+  ~ctr_sart() { drop_graph(); }
 };
 
 namespace {
@@ -520,7 +599,8 @@ const char* ctr_last_error(void) { return last_error_cstr(); }
 
 const char* ctr_version(void) { return "ctrecon_amd 0.1 (gfx950)"; }
 
-int ctr_plan_create(const ctr_geometry* geom, int device, ctr_plan** out) {
+// A plan on `device` with tables built at offset `half` (build_tables) and its stream.
+static int make_plan(const ctr_geometry* geom, int device, double half, ctr_plan** out) {
   if (!out) return fail(CTR_ERR_ARG, "out is NULL");
   *out = nullptr;
   int rc = validate_geometry(geom);
@@ -528,13 +608,22 @@ int ctr_plan_create(const ctr_geometry* geom, int device, ctr_plan** out) {
   if ((rc = set_device(device))) return rc;
   Plan* P = new Plan();
   P->device = device;
-  if ((rc = build_tables(geom, *P))) {
+  if ((rc = build_tables(geom, *P, half))) {
     free_plan(P);
     return rc;
+  }
+  if (hipStreamCreate(&P->stream) != hipSuccess) {
+    P->stream = nullptr;
+    free_plan(P);
+    return fail(CTR_ERR_HIP, "cannot create the plan's stream");
   }
   *out = new ctr_plan{P};
   set_error("");
   return CTR_OK;
+}
+
+int ctr_plan_create(const ctr_geometry* geom, int device, ctr_plan** out) {
+  return make_plan(geom, device, 0.0, out);
 }
 
 int ctr_plan_destroy(ctr_plan* plan) {
@@ -641,30 +730,17 @@ int ctr_residual(ctr_plan* plan, const float* d_x, const float* d_prj, double* o
   if (!P || !d_x || !d_prj || !out) return fail(CTR_ERR_ARG, "NULL argument");
   if (int rc0 = check_solver_method(*P)) return rc0;
   CTR_HIP(hipSetDevice(P->device));
-  const int PN = P->d.PN;
   // chunk the views so the scratch stays modest
-  const int chunk = std::min(PN, 256);
+  const int chunk = std::min(P->d.PN, 256);
   int rc = ensure_scratch_prj(*P, chunk);
   if (rc) return rc;
-  const int nb = fp_partial_blocks(*P, chunk);
-  if ((rc = ensure_partials(*P, 2 * (size_t)nb + 2))) return rc;
-  DBuf sums;
-  if ((rc = sums.alloc(2 * sizeof(double)))) return rc;
-  double rr = 0, bb = 0;
-  for (int v0 = 0; v0 < PN; v0 += chunk) {
-    const int vc = std::min(chunk, PN - v0);
-    if ((rc = fp_any(*P, d_x, P->scratch_prj, v0, 1, vc, nullptr, d_prj, nullptr,
-                        P->partials)))
-      return rc;
-    if ((rc = launch_reduce_pairs(P->partials, fp_partial_blocks(*P, vc), sums.d(), nullptr)))
-      return rc;
-    double h[2];
-    CTR_HIP(hipMemcpy(h, sums.p, sizeof(h), hipMemcpyDeviceToHost));
-    rr += h[0];
-    bb += h[1];
-  }
-  *out = bb > 0 ? std::sqrt(rr / bb) : std::sqrt(rr);
-  return CTR_OK;
+  if ((rc = ensure_partials(*P, 2 * (size_t)fp_partial_blocks(*P, chunk) + 2))) return rc;
+  size_t have = P->sums2 ? kResidualSums * sizeof(double) : 0;
+  if ((rc = ensure_bytes((void**)&P->sums2, &have, kResidualSums * sizeof(double),
+                         "residual sums")))
+    return rc;
+  return residual_run(*P, d_x, d_prj, P->scratch_prj, chunk, P->partials, P->sums2, P->stream,
+                      out);
 }
 
 int ctr_os_sart(ctr_plan* plan, float* d_x, const float* d_prj, int osNum, int iterNum,
@@ -710,8 +786,9 @@ int ctr_sart_create(ctr_plan* plan, const float* d_prj, int osNum, int bp_kind, 
     return rc;
   if (S->bf && (rc = S->xs.alloc(NV * sizeof(uint16_t)))) return rc;
   if ((rc = S->sums.alloc(2 * sizeof(double) * osNum))) return rc;
-  if ((rc = ensure_partials(*P, 2 * (size_t)fp_partial_blocks(*P, S->maxV) + 2))) return rc;
-  hipStream_t st = nullptr;
+  if ((rc = S->parts.alloc((2 * (size_t)fp_partial_blocks(*P, S->maxV) + 2) * sizeof(double))))
+    return rc;
+  hipStream_t st = P->stream;
   EventPool ev;
   if ((rc = ev.init(2))) return rc;
   CTR_HIP(hipEventRecord(ev.e[0], st));
@@ -755,12 +832,82 @@ int ctr_sart_create(ctr_plan* plan, const float* d_prj, int osNum, int bp_kind, 
 int ctr_sart_destroy(ctr_sart* S) {
   if (S) {
     (void)hipSetDevice(S->P->device);
+    (void)hipStreamSynchronize(S->P->stream);
     delete S;
   }
   return CTR_OK;
 }
 
 double ctr_sart_setup_ms(const ctr_sart* S) { return S ? S->setup_ms : 0.0; }
+
+// The launches of one sweep: per subset the fused forward projection (residual sinogram
+// and its sums), the reduction of the sums, and the back-projection update.  kev (may be
+// NULL): four events per subset around the two projector launches.
+static int sweep_launches(ctr_sart* S, float* d_x, hipStream_t st, hipEvent_t* kev) {
+  Plan* P = S->P;
+  const Dims& d = P->d;
+  const int osNum = S->osNum;
+  const size_t NV = (size_t)d.XN * d.YN * d.ZN, frame = (size_t)d.DNU * d.DNV;
+  int rc;
+  for (int s = 0; s < osNum; ++s) {
+    const int vc = (d.PN - s + osNum - 1) / osNum;
+    if (kev) CTR_HIP(hipEventRecord(kev[4 * s + 0], st));
+    // tp = (prj - A_s x) ./ RowSum (os_sart.m:24,30-31)
+    const float* rI = S->rowInv.f() + S->rowOff[s] * frame;
+    if (S->bf)
+      rc = fp_any<uint16_t, uint16_t>(*P, S->xs.h(), S->tp.h(), s, osNum, vc, st, S->prj, rI,
+                                      S->parts.d());
+    else if (S->sino16)
+      rc = fp_any<float, uint16_t>(*P, d_x, S->tp.h(), s, osNum, vc, st, S->prj, rI,
+                                   S->parts.d());
+    else
+      rc = fp_any<float, float>(*P, d_x, S->tp.f(), s, osNum, vc, st, S->prj, rI, S->parts.d());
+    if (rc) return rc;
+    if (kev) CTR_HIP(hipEventRecord(kev[4 * s + 1], st));
+    if ((rc = launch_reduce_pairs(S->parts.d(), fp_partial_blocks(*P, vc), S->sums.d() + 2 * s,
+                                  st)))
+      return rc;
+    if (kev) CTR_HIP(hipEventRecord(kev[4 * s + 2], st));
+    // x += B_s(tp) ./ ColSum (os_sart.m:32,37-40)
+    const float* cI = S->colInv.f() + NV * s;
+    rc = S->sino16 ? bp_any<uint16_t>(*P, S->tp.h(), d_x, s, osNum, vc, S->kind, st, cI, 1,
+                                      S->bf ? S->xs.h() : nullptr)
+                   : bp_any<float>(*P, S->tp.f(), d_x, s, osNum, vc, S->kind, st, cI, 1);
+    if (rc) return rc;
+    if (kev) CTR_HIP(hipEventRecord(kev[4 * s + 3], st));
+  }
+  return CTR_OK;
+}
+
+// A sweep can be replayed from a graph when every buffer its launches touch belongs to
+// the session or the caller: the direct method with the reference back projector (the
+// integral-image method and the matched adjoint use plan scratch that later calls may
+// reallocate).
+static bool graph_capturable(const ctr_sart* S) {
+  return S->P->method == CTR_METHOD_DIRECT && S->kind == CTR_BP_REFERENCE;
+}
+
+static int capture_sweep(ctr_sart* S, float* d_x, hipStream_t st) {
+  S->drop_graph();
+  CTR_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  int rc = sweep_launches(S, d_x, st, nullptr);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess || !g)
+    return fail(CTR_ERR_HIP, std::string("sweep graph capture: ") + hipGetErrorString(e));
+  S->graph = g;
+  if (hipGraphInstantiate(&S->exec, S->graph, nullptr, nullptr, 0) != hipSuccess) {
+    S->exec = nullptr;
+    S->drop_graph();
+    return fail(CTR_ERR_HIP, "sweep graph instantiation failed");
+  }
+  S->graph_x = d_x;
+  return CTR_OK;
+}
 
 int ctr_sart_sweeps(ctr_sart* S, float* d_x, int maxIter, double stop_resid, double* resid,
                     double* sweep_ms, ctr_sart_stats* stats) {
@@ -770,10 +917,12 @@ int ctr_sart_sweeps(ctr_sart* S, float* d_x, int maxIter, double stop_resid, dou
   CTR_HIP(hipSetDevice(P->device));
   const Dims& d = P->d;
   const int osNum = S->osNum;
-  const size_t NV = (size_t)d.XN * d.YN * d.ZN, frame = (size_t)d.DNU * d.DNV;
+  const size_t NV = (size_t)d.XN * d.YN * d.ZN;
   int rc;
-  hipStream_t st = nullptr;
+  hipStream_t st = P->stream;
   const bool timing = stats != nullptr || sweep_ms != nullptr;
+  // per-launch timing (stats) needs the events between the launches: plain launches then
+  const bool graph = S->use_graph && stats == nullptr && graph_capturable(S) && maxIter > 0;
   // events: [0] start, then one per sweep end; per-launch pairs for the kernel totals
   EventPool ev;
   if (timing && (rc = ev.init(1 + maxIter + 4 * osNum))) return rc;
@@ -781,42 +930,28 @@ int ctr_sart_sweeps(ctr_sart* S, float* d_x, int maxIter, double stop_resid, dou
     std::memset(stats, 0, sizeof(*stats));
     stats->setup_ms = S->setup_ms;
   }
+  if (graph && (S->exec == nullptr || S->graph_x != d_x)) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if ((rc = capture_sweep(S, d_x, st))) return rc;
+    S->capture_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
   if (timing) CTR_HIP(hipEventRecord(ev.e[0], st));
   // the caller may have changed x since the last call: refresh its BF16 shadow
   if (S->bf && (rc = launch_to_bf16_shadow(S->xs.h(), d_x, NV, st))) return rc;
-  hipEvent_t* kev = timing ? ev.e.data() + 1 + maxIter : nullptr;
+  hipEvent_t* kev = stats ? ev.e.data() + 1 + maxIter : nullptr;
+  std::vector<double> h(2 * osNum);
   int done = 0;
   for (int it = 0; it < maxIter; ++it) {
-    for (int s = 0; s < osNum; ++s) {
-      const int vc = (d.PN - s + osNum - 1) / osNum;
-      if (stats) CTR_HIP(hipEventRecord(kev[4 * s + 0], st));
-      // tp = (prj - A_s x) ./ RowSum (os_sart.m:24,30-31)
-      const float* rI = S->rowInv.f() + S->rowOff[s] * frame;
-      if (S->bf)
-        rc = fp_any<uint16_t, uint16_t>(*P, S->xs.h(), S->tp.h(), s, osNum, vc, st, S->prj, rI,
-                                        P->partials);
-      else if (S->sino16)
-        rc = fp_any<float, uint16_t>(*P, d_x, S->tp.h(), s, osNum, vc, st, S->prj, rI,
-                                     P->partials);
-      else
-        rc = fp_any<float, float>(*P, d_x, S->tp.f(), s, osNum, vc, st, S->prj, rI, P->partials);
-      if (rc) return rc;
-      if (stats) CTR_HIP(hipEventRecord(kev[4 * s + 1], st));
-      if ((rc = launch_reduce_pairs(P->partials, fp_partial_blocks(*P, vc), S->sums.d() + 2 * s,
-                                    st)))
-        return rc;
-      if (stats) CTR_HIP(hipEventRecord(kev[4 * s + 2], st));
-      // x += B_s(tp) ./ ColSum (os_sart.m:32,37-40)
-      const float* cI = S->colInv.f() + NV * s;
-      rc = S->sino16 ? bp_any<uint16_t>(*P, S->tp.h(), d_x, s, osNum, vc, S->kind, st, cI, 1,
-                                        S->bf ? S->xs.h() : nullptr)
-                     : bp_any<float>(*P, S->tp.f(), d_x, s, osNum, vc, S->kind, st, cI, 1);
-      if (rc) return rc;
-      if (stats) CTR_HIP(hipEventRecord(kev[4 * s + 3], st));
+    if (graph) {
+      CTR_HIP(hipGraphLaunch(S->exec, st));
+    } else if ((rc = sweep_launches(S, d_x, st, kev))) {
+      return rc;
     }
     if (timing) CTR_HIP(hipEventRecord(ev.e[1 + it], st));
-    std::vector<double> h(2 * osNum);
-    CTR_HIP(hipMemcpy(h.data(), S->sums.p, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    CTR_HIP(hipMemcpyAsync(h.data(), S->sums.p, h.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, st));
+    CTR_HIP(hipStreamSynchronize(st));
     double rr = 0, bb = 0;
     for (int s = 0; s < osNum; ++s) {
       rr += h[2 * s];
@@ -843,8 +978,25 @@ int ctr_sart_sweeps(ctr_sart* S, float* d_x, int maxIter, double stop_resid, dou
     done = it + 1;
     if (stop_resid > 0 && r <= stop_resid) break;
   }
-  CTR_HIP(hipDeviceSynchronize());
+  CTR_HIP(hipStreamSynchronize(st));
   if (stats) stats->sweeps = done;
+  return CTR_OK;
+}
+
+int ctr_sart_set_graph(ctr_sart* S, int enable) {
+  if (!S) return fail(CTR_ERR_ARG, "NULL session");
+  S->use_graph = enable != 0;
+  if (!S->use_graph) {
+    (void)hipSetDevice(S->P->device);
+    S->drop_graph();
+  }
+  return CTR_OK;
+}
+
+int ctr_sart_graph_info(const ctr_sart* S, int* captured, double* capture_ms) {
+  if (!S) return fail(CTR_ERR_ARG, "NULL session");
+  if (captured) *captured = S->exec != nullptr;
+  if (capture_ms) *capture_ms = S->capture_ms;
   return CTR_OK;
 }
 
@@ -893,16 +1045,30 @@ int ctr_conjgrad(ctr_plan* plan, float* d_x, const float* d_prj, int iterNum, in
   // the projectors; the CG vectors stay FP32 and the dots / scalars FP64
   const bool bf = P->storage == CTR_STORAGE_BF16;
   const bool sino16 = P->storage != CTR_STORAGE_F32;
-  DBuf r, pv, t, sino, dots, in16;
-  if ((rc = r.alloc(NV * sizeof(float)))) return rc;
-  if ((rc = pv.alloc(NV * sizeof(float)))) return rc;
-  if ((rc = t.alloc(NV * sizeof(float)))) return rc;
-  if ((rc = sino.alloc(NS * (sino16 ? sizeof(uint16_t) : sizeof(float))))) return rc;
-  if (bf && (rc = in16.alloc(NV * sizeof(uint16_t)))) return rc;
-  if ((rc = dots.alloc(2 * sizeof(double)))) return rc;
+  // the CG vectors live in the plan: the first call allocates them, later calls reuse them
+  const size_t need[5] = {NV * sizeof(float), NV * sizeof(float), NV * sizeof(float),
+                          NS * (sino16 ? sizeof(uint16_t) : sizeof(float)),
+                          bf ? NV * sizeof(uint16_t) : 0};
+  const char* what[5] = {"CG residual", "CG direction", "CG operator output",
+                         "CG sinogram", "CG BF16 input"};
+  for (int i = 0; i < 5; ++i)
+    if (need[i] && (rc = ensure_bytes(&P->cg[i], &P->cg_bytes[i], need[i], what[i]))) return rc;
+  {
+    size_t have = P->sums2 ? kResidualSums * sizeof(double) : 0;
+    if ((rc = ensure_bytes((void**)&P->sums2, &have, kResidualSums * sizeof(double),
+                           "CG dot products")))
+      return rc;
+  }
   const int nparts = 1024;
   if ((rc = ensure_partials(*P, 2 * (size_t)nparts + 2))) return rc;
-  hipStream_t st = nullptr;
+  struct V {
+    void* p;
+    float* f() const { return (float*)p; }
+    uint16_t* h() const { return (uint16_t*)p; }
+  };
+  const V r{P->cg[0]}, pv{P->cg[1]}, t{P->cg[2]}, sino{P->cg[3]}, in16{P->cg[4]};
+  double* dots = P->sums2;
+  hipStream_t st = P->stream;
   auto A = [&](const float* in, float* out) -> int {  // Back(Proj(.)) (conjgrad.m:4-6)
     if (bf) {
       int e = launch_to_bf16_shadow(in16.h(), in, NV, st);
@@ -923,10 +1089,11 @@ int ctr_conjgrad(ctr_plan* plan, float* d_x, const float* d_prj, int iterNum, in
     return bp_any<float>(*P, sino.f(), out, 0, 1, d.PN, bp_kind, st, nullptr, 0);
   };
   auto dot = [&](const float* a, const float* b, double* outv) -> int {
-    int e = launch_dot(a, b, NV, P->partials, nparts, dots.d(), st);
+    int e = launch_dot(a, b, NV, P->partials, nparts, dots, st);
     if (e) return e;
     double h[2];
-    if (hipMemcpy(h, dots.p, sizeof(h), hipMemcpyDeviceToHost) != hipSuccess)
+    if (hipMemcpyAsync(h, dots, sizeof(h), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess)
       return fail(CTR_ERR_HIP, "dot readback failed");
     *outv = h[0];
     return CTR_OK;
@@ -952,7 +1119,7 @@ int ctr_conjgrad(ctr_plan* plan, float* d_x, const float* d_prj, int iterNum, in
     rr = r1;
     if (resid) resid[it] = std::sqrt(r1);
   }
-  CTR_HIP(hipDeviceSynchronize());
+  CTR_HIP(hipStreamSynchronize(st));
   return CTR_OK;
 }
 
@@ -967,19 +1134,8 @@ int ctr_dd3_proj(float x0, float y0, float z0, int DNU, int DNV, const float* xd
                                  imgZCenter, hangs, hzPos, PN, XN, YN, ZN, dx, dz, mask);
   PlanGuard pg;
   int rc;
-  if (m == Model::Fp64Sat) {
-    // the branchless footprint model, sampled half a cell off (build_tables)
-    if ((rc = validate_geometry(&g)) || (rc = set_device(gpunum))) return rc;
-    Plan* P = new Plan();
-    P->device = gpunum;
-    if ((rc = build_tables(&g, *P, 0.5))) {
-      free_plan(P);
-      return rc;
-    }
-    pg.p = new ctr_plan{P};
-  } else if ((rc = ctr_plan_create(&g, gpunum, &pg.p))) {
-    return rc;
-  }
+  // prjMode 2: the branchless footprint model, sampled half a cell off (build_tables)
+  if ((rc = make_plan(&g, gpunum, m == Model::Fp64Sat ? 0.5 : 0.0, &pg.p))) return rc;
   if (m == Model::PseudoDD && (rc = ctr_plan_set_method(pg.p, CTR_METHOD_PSEUDO_DD))) return rc;
   const size_t NV = (size_t)XN * YN * ZN, NS = (size_t)DNU * DNV * PN;
   DBuf dv, dp;
@@ -1036,7 +1192,7 @@ int ctr_malloc(void** d_ptr, unsigned long long bytes, int device) {
   if (!d_ptr) return fail(CTR_ERR_ARG, "d_ptr is NULL");
   int rc = set_device(device);
   if (rc) return rc;
-  if (hipMalloc(d_ptr, bytes ? bytes : 1) != hipSuccess) {
+  if (dev_malloc(d_ptr, bytes) != hipSuccess) {
     *d_ptr = nullptr;
     return fail(CTR_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
   }
@@ -1080,6 +1236,11 @@ int ctr_device_sub(float* d_out, const float* d_a, const float* d_b, unsigned lo
 int ctr_device_axpy(float* d_y, const float* d_x, double a, unsigned long long n, void* stream) {
   if (!d_y || !d_x) return fail(CTR_ERR_ARG, "NULL pointer");
   return launch_axpy(d_y, d_x, a, n, (hipStream_t)stream);
+}
+
+int ctr_debug_fail_alloc_after(long long n) {
+  g_alloc_countdown.store(n < 0 ? -1 : n);
+  return CTR_OK;
 }
 
 int ctr_synchronize(int device) {

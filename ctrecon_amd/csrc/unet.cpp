@@ -56,7 +56,10 @@ struct Buf {
     if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
-    if (hipMalloc(&p, b ? b : 1) != hipSuccess) return fail(CTR_ERR_NOMEM, "network buffer");
+    if (dev_malloc(&p, b) != hipSuccess) {
+      p = nullptr;
+      return fail(CTR_ERR_NOMEM, "network buffer");
+    }
     bytes = b;
     return CTR_OK;
   }

@@ -283,7 +283,7 @@ def warm_start_os_sart(plan: Plan, net: UNet, d_b: DeviceArray, d_x: DeviceArray
         if stage_fallbacks[k]:
             d_x.copy_from(keep)
         reached = r5[-1] <= stop_resid and not stage_fallbacks[k]
-    if not reached:
+    if not reached and max_sweeps > done:
         r = S.sweeps(d_x, max_sweeps - done, stop_resid)[0]
         res_after += list(r)
         done += len(r)

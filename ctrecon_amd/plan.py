@@ -95,15 +95,31 @@ class Sart:
         self.handle = h.value
         self.setup_ms = float(lib().ctr_sart_setup_ms(self.handle))
 
-    def sweeps(self, d_x: DeviceArray, n: int, stop_resid=0.0):
-        """Runs up to n sweeps; returns (residuals, sweep_ms, stats) of those run."""
+    def sweeps(self, d_x: DeviceArray, n: int, stop_resid=0.0, stats=False):
+        """Runs up to n sweeps; returns (residuals, sweep_ms, stats) of those run.  With
+        stats=True the launches are timed one by one (plain launches, per-kernel totals
+        in the stats dict); otherwise sweeps replay a captured graph where the session
+        allows it and the stats dict is None."""
         res = np.zeros(max(n, 1), dtype=np.float64)
         ms = np.zeros(max(n, 1), dtype=np.float64)
         st = CtrSartStats()
         check(lib().ctr_sart_sweeps(self.handle, d_x.ptr, n, float(stop_resid), res.ctypes.data,
-                                    ms.ctypes.data, ctypes.byref(st)))
+                                    ms.ctypes.data, ctypes.byref(st) if stats else None))
+        if not stats:
+            k = _sweeps_run(res, n, stop_resid)
+            return res[:k], ms[:k], None
         k = st.sweeps
         return res[:k], ms[:k], {f: getattr(st, f) for f, _ in CtrSartStats._fields_}
+
+    def set_graph(self, enable: bool):
+        """Graph replay of the sweeps on (default) or off."""
+        check(lib().ctr_sart_set_graph(self.handle, 1 if enable else 0))
+
+    def graph_info(self):
+        """(captured, capture_ms) of the session's sweep graph."""
+        c, ms = ctypes.c_int(), ctypes.c_double()
+        check(lib().ctr_sart_graph_info(self.handle, ctypes.byref(c), ctypes.byref(ms)))
+        return bool(c.value), ms.value
 
     def close(self):
         if getattr(self, "handle", None):
@@ -115,6 +131,18 @@ class Sart:
             self.close()
         except Exception:
             pass
+
+
+def _sweeps_run(res, n, stop_resid):
+    """Number of sweeps a call ran: n, or fewer when the stopping rule ended it (the
+    first sweep whose residual is <= stop_resid)."""
+    if n <= 0:
+        return 0
+    if stop_resid > 0:
+        hit = np.nonzero(res[:n] <= stop_resid)[0]
+        if hit.size:
+            return int(hit[0]) + 1
+    return n
 
 
 def to_bf16(d_src: DeviceArray, d_dst: DeviceArray = None, stream=None) -> DeviceArray:
@@ -230,18 +258,22 @@ class Plan:
         return res[:iterNum]
 
     def os_sart_run(self, d_x: DeviceArray, d_prj: DeviceArray, osNum: int, maxIter: int,
-                    kind="ref", stop_resid=0.0):
+                    kind="ref", stop_resid=0.0, stats=True):
         """OS-SART with a stopping rule and HIP-event timing; returns
-        (residuals, sweep_ms, stats) for the sweeps actually run."""
+        (residuals, sweep_ms, stats) for the sweeps actually run.  stats=True times every
+        launch (plain launches); stats=False replays a captured sweep graph where the
+        session allows it and returns stats None."""
         res = np.zeros(max(maxIter, 1), dtype=np.float64)
         ms = np.zeros(max(maxIter, 1), dtype=np.float64)
         st = CtrSartStats()
         check(lib().ctr_os_sart_run(self.handle, d_x.ptr, d_prj.ptr, osNum, maxIter,
                                     bp_kind_code(kind), float(stop_resid), res.ctypes.data,
-                                    ms.ctypes.data, ctypes.byref(st)))
+                                    ms.ctypes.data, ctypes.byref(st) if stats else None))
+        if not stats:
+            n = _sweeps_run(res, maxIter, stop_resid)
+            return res[:n], ms[:n], None
         n = st.sweeps
-        stats = {k: getattr(st, k) for k, _ in CtrSartStats._fields_}
-        return res[:n], ms[:n], stats
+        return res[:n], ms[:n], {k: getattr(st, k) for k, _ in CtrSartStats._fields_}
 
     def cone_voxels(self, view_begin=0, view_stride=1, view_count=None):
         """Masked voxels inside each view's cone (host computation)."""

@@ -194,14 +194,23 @@ int ctr_os_sart_run(ctr_plan* plan, float* d_x, const float* d_prj, int osNum, i
  * sinogram (os_sart.m:14-18) computed once by ctr_sart_create, then any number of
  * ctr_sart_sweeps calls on an iterate that the caller may modify in between (the
  * learned warm start does).  The session uses the plan's method and storage at
- * creation; d_prj must stay valid until ctr_sart_destroy.  Sessions of one plan must
- * not run concurrently (they share the plan's reduction scratch). */
+ * creation; d_prj must stay valid until ctr_sart_destroy.  Each session owns its
+ * reduction scratch; every session of a plan runs on the plan's stream, so calls on one
+ * plan must come from one thread at a time. */
 typedef struct ctr_sart ctr_sart;
 int ctr_sart_create(ctr_plan* plan, const float* d_prj, int osNum, int bp_kind,
                     ctr_sart** out);
-/* Sweeps as in ctr_os_sart_run (stats->setup_ms is the session's set-up time). */
+/* Sweeps as in ctr_os_sart_run (stats->setup_ms is the session's set-up time).  With
+ * the direct method and the reference back projector, and stats == NULL, the first call
+ * for an iterate d_x captures one sweep (3 x osNum launches on the plan's stream) into a
+ * HIP graph and every sweep replays it; a new d_x is captured again.  With stats, the
+ * per-launch timing events need plain launches. */
 int ctr_sart_sweeps(ctr_sart* sart, float* d_x, int maxIter, double stop_resid, double* resid,
                     double* sweep_ms, ctr_sart_stats* stats);
+/* Graph replay on (default) or plain launches for the session's later sweeps. */
+int ctr_sart_set_graph(ctr_sart* sart, int enable);
+/* Whether a sweep graph is held, and the host time its capture + instantiation took. */
+int ctr_sart_graph_info(const ctr_sart* sart, int* captured, double* capture_ms);
 double ctr_sart_setup_ms(const ctr_sart* sart);
 int ctr_sart_destroy(ctr_sart* sart);
 
@@ -214,7 +223,9 @@ int ctr_cone_voxels(ctr_plan* plan, int view_begin, int view_stride, int view_co
 
 /* CG, iterativeRecon/conjgrad.m:1-35, on the device: A = Back(Proj(.)),
  * b = Back(prj); dot products and step scalars in FP64.  resid (host, may be
- * NULL) receives ||r_k|| per iteration.  Synchronous. */
+ * NULL) receives ||r_k|| per iteration.  The CG vectors are kept in the plan: the first
+ * call allocates them, later calls of the same storage allocate nothing.  Runs on the
+ * plan's stream.  Synchronous. */
 int ctr_conjgrad(ctr_plan* plan, float* d_x, const float* d_prj, int iterNum, int bp_kind,
                  double* resid);
 
@@ -227,7 +238,9 @@ int ctr_residual(ctr_plan* plan, const float* d_x, const float* d_prj, double* o
  * destroyed).  ctr_batch_create computes every scan's row / column sums and tunes the
  * launch shapes on the batch (persistent grid sizes, forward work mapping, z groups per
  * lane of the back projector, contiguous / interleaved / one-per-item work assignment,
- * chosen by timing); each subset step then runs as one
+ * chosen by timing; the choice is cached in a file keyed by the device and the batch's
+ * geometry -- $CTRECON_TUNE_CACHE, else ~/.cache/ctrecon_amd/batch_tuning.txt -- and a
+ * later batch with the same key takes it from there); each subset step then runs as one
  * forward, one reduction and one back-projection launch per back-projection class over all
  * scans, and a whole sweep is captured once into a HIP graph and replayed.  Reference back projector
  * only. */
@@ -244,6 +257,7 @@ typedef struct ctr_batch_info {
   double tune_ms;     /* autotuning (host wall time) */
   double capture_ms;  /* graph capture + instantiation (host wall time; 0 before) */
   long long fp_items, bp_items;  /* work items per sweep */
+  int from_cache;     /* 1: the launch shapes came from the tuning cache (tune_ms: lookup) */
 } ctr_batch_info;
 int ctr_batch_create(ctr_plan* const* plans, int nscan, float* const* d_x,
                      const float* const* d_prj, int osNum, int bp_kind, ctr_batch** out);
@@ -311,6 +325,11 @@ int ctr_bf16_to_f32(float* d_dst, const unsigned short* d_src, unsigned long lon
 int ctr_device_sub(float* d_out, const float* d_a, const float* d_b, unsigned long long n,
                    void* stream);
 int ctr_device_axpy(float* d_y, const float* d_x, double a, unsigned long long n, void* stream);
+
+/* Allocation-failure injection for tests (build addition): after n more successful
+ * device allocations by the library, the next one fails with CTR_ERR_NOMEM (and the
+ * injection turns itself off); n < 0 turns it off. */
+int ctr_debug_fail_alloc_after(long long n);
 
 /* Message for the last failing call on this thread ("" if none). */
 const char* ctr_last_error(void);

@@ -1,0 +1,156 @@
+"""The batched form at its configured sizes: 16 helical scans of uneven size (seeds
+100-115: volumes U{256..512}^2 x U{64..192}, voxels U(0.5, 1.0) mm, pitch U(0.5, 1.5),
+2-4 turns; 888 x 64 detector, 984 views/turn), against the FP64 oracle.
+
+* Every scan's forward projection on sampled views of its first OS-SART subset (views 0
+  and 20 of 20 subsets) within 1e-5 of the oracle.
+* The full batch with BF16 storage (tuned launch shapes, graph replay): two sweeps lower
+  every scan's residual, and agree with each scan's own OS-SART session.
+* One batched subset update per scan against the oracle's FP64 OS-SART step, on four
+  scans (the smallest and largest volumes, the lowest and highest pitch) cut to 20
+  views each: within 1e-5 with FP32 storage, and within 2e-2 with BF16 storage (the
+  iterate read in dithered BF16)."""
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+SEEDS = list(range(100, 116))
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _scan(seed):
+    """Geometry and phantom of one scan of the batch (as bench.py's batched workload)."""
+    from ctrecon_amd import geometry as G
+    from ctrecon_amd import phantom as P
+
+    rng = np.random.default_rng(seed)
+    nxy = int(rng.integers(256, 513)) // 4 * 4
+    nz = int(rng.integers(64, 193)) // 4 * 4
+    vox = float(rng.uniform(0.5, 1.0))
+    pitch = float(rng.uniform(0.5, 1.5))
+    turns = int(rng.integers(2, 5))
+    cfg = G.helical_config(sid=541, sdd=949, DNU=888, DNV=64, col_width=1.0239,
+                           col_height=1.096439, views_per_turn=984, turns=turns,
+                           feed_mm_per_turn=pitch * 64 * 0.625, N_xy=nxy, N_z=nz,
+                           voxel_xy=vox, voxel_z=vox, col_height_at_iso=0.625)
+    g = G.dd3_geometry(cfg, mask=G.fov_mask(nxy))
+    return g, P.ellipsoid_mixture(g, seed=seed, n=12), pitch
+
+
+@pytest.fixture(scope="module")
+def scans():
+    return {s: _scan(s) for s in SEEDS}
+
+
+def test_config4_forward_sampled_views(scans, oracle, gpu_plan_factory):
+    for seed in SEEDS:
+        g, vol, pitch = scans[seed]
+        plan = gpu_plan_factory(g)
+        for p in (0, 20):
+            got = plan.forward(vol, [p])
+            ref = oracle.fp(g.subset([p]), vol, "direct")
+            err = rel_l2(got, ref)
+            _say(f"  scan {seed} ({g.XN}x{g.YN}x{g.ZN}, pitch {pitch:.2f}) view {p}: "
+                 f"fp rel-L2 {err:.3e}")
+            assert err <= 1e-5
+
+
+def test_config4_batch_sweeps(scans, gpu_plan_factory):
+    from ctrecon_amd.batch import Batch
+    from ctrecon_amd.plan import DeviceArray, Plan, Sart
+
+    plans, xs, bs = [], [], []
+    for seed in SEEDS:
+        g, vol, _ = scans[seed]
+        p = Plan(g, 0)
+        p.set_storage("bf16")
+        b = DeviceArray(g.prj_shape)
+        p.fp(DeviceArray.from_host(vol), b)
+        x = DeviceArray(g.vol_shape)
+        x.zero()
+        plans.append(p)
+        xs.append(x)
+        bs.append(b)
+    _say(f"[config 4] {len(plans)} scans, {sum(p.g.PN for p in plans)} views")
+    B = Batch(plans, xs, bs, 20)
+    res, ms = B.sweeps(2)
+    _say(f"  tuning {B.info()}; sweep ms {ms}")
+    B.close()
+    for i, seed in enumerate(SEEDS):
+        _say(f"  scan {seed}: residuals {res[:, i]}")
+        assert res[1, i] < res[0, i]
+    # each scan's own session from zero gives the same sweeps (FP32 rounding of the
+    # regrouped z arithmetic in tuned shapes)
+    for i, seed in enumerate(SEEDS[:: 5]):
+        k = SEEDS.index(seed)
+        x = DeviceArray(plans[k].g.vol_shape)
+        x.zero()
+        S = Sart(plans[k], bs[k], 20)
+        r, _, _ = S.sweeps(x, 2)
+        S.close()
+        e = rel_l2(xs[k].get(), x.get())
+        _say(f"  scan {seed}: batch vs session iterate rel-L2 {e:.3e}, residuals {r}")
+        assert e <= 1e-4
+        np.testing.assert_allclose(res[:, k], r, rtol=1e-4)
+
+
+@pytest.fixture(scope="module")
+def subset_refs(scans, oracle):
+    """Four scans (smallest and largest volume, lowest and highest pitch) cut to 20 views
+    spread over the scan, x0 = half the phantom, b = the oracle's projection: the oracle's
+    FP64 OS-SART step and its column sums."""
+    vox = {s: scans[s][0].XN * scans[s][0].YN * scans[s][0].ZN for s in SEEDS}
+    pit = {s: scans[s][2] for s in SEEDS}
+    pick = []
+    for s in (min(vox, key=vox.get), max(vox, key=vox.get), min(pit, key=pit.get),
+              max(pit, key=pit.get)):
+        if s not in pick:
+            pick.append(s)
+    out = []
+    for seed in pick:
+        g, vol, pitch = scans[seed]
+        gs = g.subset(list(range(0, g.PN, g.PN // 20))[:20])
+        b = oracle.fp(gs, vol, "direct").astype(np.float32)
+        x0 = (0.5 * vol).astype(np.float32)
+        xr, _ = oracle.os_sart(gs, b, x0=x0, osNum=1, iterNum=1)
+        col = oracle.bp(gs, np.ones(gs.prj_shape, np.float32), "direct")
+        out.append((seed, gs, b, x0, xr - x0, col >= 0.01 * col.max()))
+        _say(f"  oracle step for scan {seed} done")
+    return out
+
+
+@pytest.mark.parametrize("storage,tol", [("f32", 1e-5), ("bf16", 2e-2)])
+def test_config4_subset_update_vs_oracle(storage, tol, subset_refs, gpu_plan_factory):
+    """Voxels that a view's cone only grazes divide two small sums (os_sart.m:38-39): the
+    bar holds where the column sum is at least 1% of the largest, and 10x the bar on every
+    voxel."""
+    from ctrecon_amd.batch import Batch
+    from ctrecon_amd.plan import DeviceArray, Plan
+
+    plans, xs, bs = [], [], []
+    for seed, gs, b, x0, _, _ in subset_refs:
+        p = Plan(gs, 0)
+        p.set_storage(storage)
+        plans.append(p)
+        xs.append(DeviceArray.from_host(x0))
+        bs.append(DeviceArray.from_host(b))
+    _say(f"[config 4 vs oracle] scans {[r[0] for r in subset_refs]}, storage {storage}")
+    B = Batch(plans, xs, bs, 1)
+    B.sweeps(1)
+    B.close()
+    for k, (seed, gs, b, x0, upd_ref, well) in enumerate(subset_refs):
+        upd = xs[k].get().astype(np.float64) - x0
+        err = rel_l2(upd[well], upd_ref[well])
+        err_all = rel_l2(upd, upd_ref)
+        _say(f"  scan {seed}: update rel-L2 {err:.3e} ({well.mean():.1%} of the voxels), "
+             f"{err_all:.3e} on all")
+        assert np.linalg.norm(upd_ref) > 0
+        assert err <= tol
+        assert err_all <= 10 * tol
