@@ -23,7 +23,6 @@
 #include <cstdint>
 #include <cstring>
 #include <string>
-#include <vector>
 
 #include "ctrecon.h"
 #include "mex.h"
@@ -127,18 +126,20 @@ void back(mxArray* plhs[], const mxArray* const* a) {
                      sino, dx, dz, mask, gpu, squared, mode));
 }
 
-// Owns device memory for the loop actions.
-struct DevBuf {
-  void* p = nullptr;
-  ~DevBuf() {
-    if (p) ctr_free(p);
-  }
-};
-
-struct PlanOwner {
-  ctr_plan* p = nullptr;
-  ~PlanOwner() {
-    if (p) ctr_plan_destroy(p);
+// Device resources of a loop action.  MATLAB's mexErrMsgIdAndTxt leaves the MEX function
+// without running C++ destructors, so they are released explicitly before any error is
+// raised; the output arrays are created first (MATLAB frees those itself on an error).
+struct LoopRes {
+  ctr_plan* plan = nullptr;
+  void* x = nullptr;
+  void* prj = nullptr;
+  void release() {
+    if (x) ctr_free(x);
+    if (prj) ctr_free(prj);
+    if (plan) ctr_plan_destroy(plan);
+    x = nullptr;
+    prj = nullptr;
+    plan = nullptr;
   }
 };
 
@@ -157,26 +158,31 @@ void loop(int nlhs, mxArray* plhs[], int nrhs, const mxArray* const* a, bool sar
   if (iterNum < 0) fail("DD3_GPU:size", "iterNum must be >= 0");
   ctr_geometry geo{g.x0, g.y0, g.z0, g.DNU, g.DNV, g.xds, g.yds, g.zds, g.xoff, g.yoff, g.zoff,
                    g.ang, g.zsh, g.PN, g.XN, g.YN, g.ZN, dx, dz, mask};
-  PlanOwner plan;
-  check(ctr_plan_create(&geo, gpu, &plan.p));
-  DevBuf dx_, dp_;
-  check(ctr_malloc(&dx_.p, nvox(g) * sizeof(float), gpu));
-  check(ctr_malloc(&dp_.p, nsino(g) * sizeof(float), gpu));
-  check(ctr_memcpy_h2d(dx_.p, x0, nvox(g) * sizeof(float)));
-  check(ctr_memcpy_h2d(dp_.p, prj, nsino(g) * sizeof(float)));
-  std::vector<double> res(iterNum > 0 ? iterNum : 1, 0.0);
-  if (sart)
-    check(ctr_os_sart(plan.p, (float*)dx_.p, (const float*)dp_.p, osNum, iterNum, kind,
-                      res.data()));
-  else
-    check(ctr_conjgrad(plan.p, (float*)dx_.p, (const float*)dp_.p, iterNum, kind, res.data()));
   const mwSize dims[3] = {(mwSize)g.ZN, (mwSize)g.XN, (mwSize)g.YN};
   plhs[0] = mxCreateNumericArray(3, dims, mxSINGLE_CLASS, mxREAL);
-  check(ctr_memcpy_d2h(mxGetData(plhs[0]), dx_.p, nvox(g) * sizeof(float)));
-  if (nlhs > 1) {
-    plhs[1] = mxCreateDoubleMatrix(1, iterNum, mxREAL);
-    if (iterNum > 0) std::memcpy(mxGetPr(plhs[1]), res.data(), iterNum * sizeof(double));
-  }
+  // the residuals: the second output, or a scratch array when it is not requested
+  mxArray* res = mxCreateDoubleMatrix(1, iterNum, mxREAL);
+  if (nlhs > 1) plhs[1] = res;
+  LoopRes R;
+  auto ok = [&](int rc) {
+    if (rc != CTR_OK) {
+      const std::string msg = ctr_last_error();
+      R.release();
+      fail("DD3_GPU:ctr", msg);
+    }
+  };
+  ok(ctr_plan_create(&geo, gpu, &R.plan));
+  ok(ctr_malloc(&R.x, nvox(g) * sizeof(float), gpu));
+  ok(ctr_malloc(&R.prj, nsino(g) * sizeof(float), gpu));
+  ok(ctr_memcpy_h2d(R.x, x0, nvox(g) * sizeof(float)));
+  ok(ctr_memcpy_h2d(R.prj, prj, nsino(g) * sizeof(float)));
+  if (sart)
+    ok(ctr_os_sart(R.plan, (float*)R.x, (const float*)R.prj, osNum, iterNum, kind, mxGetPr(res)));
+  else
+    ok(ctr_conjgrad(R.plan, (float*)R.x, (const float*)R.prj, iterNum, kind, mxGetPr(res)));
+  ok(ctr_memcpy_d2h(mxGetData(plhs[0]), R.x, nvox(g) * sizeof(float)));
+  R.release();
+  if (nlhs <= 1) mxDestroyArray(res);
 }
 
 }  // namespace

@@ -7,4 +7,4 @@ mkdir -p "$HERE/_build"
 g++ -std=c++17 -O1 -Wall -I"$ROOT/include" -I"$HERE" \
   "$ROOT/ctrecon_amd/mex/DD3_GPU_mex.cpp" "$HERE/mex_driver.cpp" \
   -L"$ROOT/ctrecon_amd" -lctrecon_amd -Wl,-rpath,'$ORIGIN/../../../ctrecon_amd' \
-  -o "$HERE/_build/mex_driver"
+  -ldl -o "$HERE/_build/mex_driver"

@@ -1,7 +1,10 @@
 // Minimal stand-in for the MATLAB C matrix / MEX API, enough to compile and drive
 // the DD3_GPU gateway without MATLAB.  TEST INFRASTRUCTURE ONLY.
-// mexErrMsgIdAndTxt throws MexError so a driver can check error identifiers.
+// mexErrMsgIdAndTxt throws MexError so a driver can check error identifiers; when a
+// driver sets mex_error_jump(), it longjmps there instead, leaving the gateway the way
+// MATLAB does (no C++ destructors run).
 #pragma once
+#include <csetjmp>
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
@@ -74,12 +77,24 @@ inline mxArray* mxCreateString(const char* s) {
   return a;
 }
 inline void mxDestroyArray(mxArray* a) { delete a; }
+inline std::jmp_buf*& mex_error_jump() {
+  static std::jmp_buf* j = nullptr;
+  return j;
+}
+inline std::string& mex_error_text() {
+  static std::string s;
+  return s;
+}
 [[noreturn]] inline void mexErrMsgIdAndTxt(const char* id, const char* fmt, ...) {
   char buf[1024];
   va_list ap;
   va_start(ap, fmt);
   std::vsnprintf(buf, sizeof buf, fmt, ap);
   va_end(ap);
+  if (mex_error_jump()) {
+    mex_error_text() = std::string(id) + " " + buf;
+    std::longjmp(*mex_error_jump(), 1);
+  }
   throw MexError(id, buf);
 }
 

@@ -1,14 +1,54 @@
 // Drives the DD3_GPU gateway through the stand-in MATLAB API.  TEST INFRASTRUCTURE.
 //   mex_driver errors        -> one line per bad call: the error identifier raised
 //   mex_driver run <dir>     -> reads geometry/inputs written by the test, calls
-//                               'Proj', 'Back' and 'OSSART', writes the outputs
+//                               'Proj', 'Back', 'OSSART' and 'CG', writes the outputs
+//   mex_driver leak <dir>    -> 'OSSART' with the k-th device allocation failing, for
+//                               k = 0, 1, ... until the call succeeds, leaving the gateway
+//                               the way MATLAB does (longjmp, no destructors); prints the
+//                               plans and device buffers still alive after each call
+#include <dlfcn.h>
+
 #include <cstdio>
 #include <fstream>
 #include <iostream>
+#include <set>
 #include <string>
 #include <vector>
 
+#include "ctrecon.h"
 #include "mex.h"
+
+// Live plans and device buffers: the gateway's calls of these C-ABI functions land here
+// (the executable's definitions come first) and are forwarded to the library.  Only
+// handles created through here are counted (the library's own internal calls may or may
+// not pass through).
+static std::set<void*> g_plans, g_bufs;
+template <typename F>
+static F next_fn(const char* name) {
+  return reinterpret_cast<F>(dlsym(RTLD_NEXT, name));
+}
+extern "C" int ctr_plan_create(const ctr_geometry* g, int device, ctr_plan** out) {
+  static auto f = next_fn<int (*)(const ctr_geometry*, int, ctr_plan**)>("ctr_plan_create");
+  const int rc = f(g, device, out);
+  if (rc == CTR_OK && out && *out) g_plans.insert(*out);
+  return rc;
+}
+extern "C" int ctr_plan_destroy(ctr_plan* p) {
+  static auto f = next_fn<int (*)(ctr_plan*)>("ctr_plan_destroy");
+  g_plans.erase(p);
+  return f(p);
+}
+extern "C" int ctr_malloc(void** p, unsigned long long bytes, int device) {
+  static auto f = next_fn<int (*)(void**, unsigned long long, int)>("ctr_malloc");
+  const int rc = f(p, bytes, device);
+  if (rc == CTR_OK && p && *p) g_bufs.insert(*p);
+  return rc;
+}
+extern "C" int ctr_free(void* p) {
+  static auto f = next_fn<int (*)(void*)>("ctr_free");
+  g_bufs.erase(p);
+  return f(p);
+}
 
 static mxArray* f32(float v) {
   const mwSize d[2] = {1, 1};
@@ -146,6 +186,40 @@ int main(int argc, char** argv) {
   const std::string dir = argv[2];
   auto mask = readv<unsigned char>(dir + "/mask.u8");
   auto sc = readv<float>(dir + "/scalars.f32");
+  if (mode == "leak") {
+    for (int k = 0; k < 64; ++k) {
+      Call c;
+      geometry(c, "OSSART", dir);
+      c.rhs.push_back(vecf(std::vector<float>(readv<float>(dir + "/vol.f32").size(), 0.f)));
+      c.rhs.push_back(vecf(readv<float>(dir + "/sino.f32")));
+      c.rhs.push_back(f32(sc[6]));
+      c.rhs.push_back(f32(sc[7]));
+      c.rhs.push_back(veci8(mask));
+      c.rhs.push_back(f64(0));
+      c.rhs.push_back(i32(4));
+      c.rhs.push_back(i32(1));
+      c.rhs.push_back(i32(0));
+      out[0] = out[1] = nullptr;
+      std::jmp_buf jb;
+      ctr_debug_fail_alloc_after(k);
+      volatile int failed = 0;
+      if (setjmp(jb) == 0) {
+        mex_error_jump() = &jb;
+        mexFunction(2, out, (int)c.rhs.size(), (const mxArray**)c.rhs.data());
+      } else {
+        failed = 1;
+      }
+      mex_error_jump() = nullptr;
+      ctr_debug_fail_alloc_after(-1);
+      std::cout << "leak k=" << k << " error=" << failed << " plans=" << g_plans.size()
+                << " buffers=" << g_bufs.size()
+                << (failed ? " " + mex_error_text() : std::string()) << "\n";
+      if (out[0]) mxDestroyArray(out[0]);
+      if (out[1]) mxDestroyArray(out[1]);
+      if (!failed) return 0;
+    }
+    return 1;
+  }
   {
     Call c;
     geometry(c, "Proj", dir);
@@ -197,5 +271,23 @@ int main(int argc, char** argv) {
     mxDestroyArray(out[0]);
     mxDestroyArray(out[1]);
   }
-  return 0;
+  {  // the argument list conjgrad.m passes
+    Call c;
+    geometry(c, "CG", dir);
+    c.rhs.push_back(vecf(std::vector<float>(readv<float>(dir + "/vol.f32").size(), 0.f)));
+    c.rhs.push_back(vecf(readv<float>(dir + "/sino.f32")));
+    c.rhs.push_back(f32(sc[6]));
+    c.rhs.push_back(f32(sc[7]));
+    c.rhs.push_back(veci8(mask));
+    c.rhs.push_back(f64(0));
+    c.rhs.push_back(i32(3));  // iterNum
+    c.rhs.push_back(i32(0));  // reference back projector
+    std::string r = run(c, 1, out);
+    std::cout << "CG " << r << "\n";
+    if (r != "ok") return 1;
+    writev(dir + "/cg_out.f32", mxGetData(out[0]), mxGetNumberOfElements(out[0]) * 4);
+    mxDestroyArray(out[0]);
+  }
+  std::cout << "live plans=" << g_plans.size() << " buffers=" << g_bufs.size() << "\n";
+  return g_plans.size() == 0 && g_bufs.size() == 0 ? 0 : 1;
 }

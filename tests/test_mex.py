@@ -57,3 +57,84 @@ def test_gateway_matches_plan(tmp_path, oracle, gpu_plan_factory):
     sart = np.fromfile(tmp_path / "sart_out.f32", np.float32).reshape(g.vol_shape)
     xr, _ = oracle.os_sart(g, y, osNum=4, iterNum=2)
     assert rel_l2(sart, xr) < 1e-4
+    # the argument list conjgrad.m passes (3 iterations), against the oracle's CG
+    cg = np.fromfile(tmp_path / "cg_out.f32", np.float32).reshape(g.vol_shape)
+    xc, _ = oracle.conjgrad(g, y, iterNum=3)
+    assert rel_l2(cg, xc) < 1e-4
+    assert "live plans=0 buffers=0" in out.stdout
+
+
+def _write_case(tmp_path):
+    cfg, g = tiny()
+    vol, y = volume(g, seed=21), sinogram(g, seed=22)
+    np.array([g.DNU, g.DNV, g.PN, g.XN, g.YN, g.ZN], np.int32).tofile(tmp_path / "sizes.i32")
+    np.array([g.x0, g.y0, g.z0, g.imgXCenter, g.imgYCenter, g.imgZCenter, g.dx, g.dz],
+             np.float32).tofile(tmp_path / "scalars.f32")
+    for name in ("xds", "yds", "zds", "angles", "zshifts"):
+        np.asarray(getattr(g, name), np.float32).tofile(tmp_path / f"{name}.f32")
+    g.mask.astype(np.uint8).tofile(tmp_path / "mask.u8")
+    vol.tofile(tmp_path / "vol.f32")
+    y.tofile(tmp_path / "sino.f32")
+    return g
+
+
+@pytest.mark.gpu
+def test_gateway_releases_everything_on_errors(tmp_path, gpu_plan_factory):
+    """'OSSART' with each device allocation failing in turn, the gateway left the way
+    MATLAB leaves it on mexErrMsgIdAndTxt (no C++ destructors): every call that fails
+    raises DD3_GPU:ctr and leaves no plan and no device buffer alive."""
+    print("[mex gateway] allocation failures", flush=True)
+    _write_case(tmp_path)
+    out = subprocess.run([_driver(), "leak", str(tmp_path)], capture_output=True, text=True)
+    print(out.stdout, out.stderr, flush=True)
+    assert out.returncode == 0
+    lines = [l for l in out.stdout.splitlines() if l.startswith("leak ")]
+    failed = [l for l in lines if " error=1 " in l]
+    assert len(failed) >= 10  # the plan's tables, the two buffers, the session's sums
+    for l in failed:
+        assert "DD3_GPU:ctr" in l
+    for l in lines:
+        assert " plans=0 buffers=0" in l, l
+    assert " error=0 " in lines[-1]
+
+
+def test_matlab_wrappers_call_the_gateway_with_its_argument_lists():
+    """os_sart.m / conjgrad.m keep the reference signatures (iterativeRecon/os_sart.m:1,
+    iterativeRecon/conjgrad.m:1) and pass the gateway's 'OSSART' (27) and 'CG' (26)
+    argument lists: the action, the 17 geometry values of ctr_dd3_args.m, then the
+    image, sinogram, dx, dz, mask, gpu and the loop's counts and back projector."""
+    import re
+
+    root = os.path.join(os.path.dirname(HERE), "ctrecon_amd", "matlab")
+
+    def call_args(src, action):
+        src = "\n".join(l for l in src.splitlines() if not l.lstrip().startswith("%"))
+        m = re.search(r"DD3_GPU\('" + action + r"',(.*?)\);", src, re.S)
+        body = m.group(1).replace("...", " ")
+        depth, parts, cur = 0, [], ""
+        for ch in body:
+            if ch in "([{":
+                depth += 1
+            elif ch in ")]}":
+                depth -= 1
+            if ch == "," and depth == 0:
+                parts.append(cur.strip())
+                cur = ""
+            else:
+                cur += ch
+        parts.append(cur.strip())
+        return parts
+
+    helper = open(os.path.join(root, "ctr_dd3_args.m")).read()
+    geo = re.search(r"geo = \{(.*?)\};", helper, re.S).group(1).replace("...", " ")
+    assert len([p for p in geo.split(",") if p.strip()]) == 17
+    sart = open(os.path.join(root, "os_sart.m")).read()
+    assert sart.startswith("function [reconImg, resid] = os_sart(initImg, prj, cfg, mask, "
+                           "osNum, iterNum)")
+    a = call_args(sart, "OSSART")
+    assert a[0] == "geo{:}" and len(a) - 1 + 17 == 26  # + the action name = 27
+    cg = open(os.path.join(root, "conjgrad.m")).read()
+    assert cg.startswith("function [reconImg, resid] = conjgrad(initImg, prj, cfg, mask, "
+                         "iterNum)")
+    a = call_args(cg, "CG")
+    assert a[0] == "geo{:}" and len(a) - 1 + 17 == 25  # + the action name = 26
