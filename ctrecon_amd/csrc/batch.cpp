@@ -76,7 +76,8 @@ struct ctr_batch {
   std::vector<int> clsScans[5];
   Mem bpItems[5], tuneBpItems[5];
   std::vector<int> bpTotal[5];
-  int winZ = 0;  // tallest column of the windowed class
+  int winZ = 0;    // tallest column of the windowed class
+  int winDNV = 0;  // most detector rows of a scan of the windowed class
   bool colw = false;
   // zg: z groups per lane for every scan (1..4), 0 for each scan's own ceil(ZN / 64), or
   // kZgWindow for the windowed form (scans with columns taller than kWinMaxZ: their own)
@@ -117,10 +118,14 @@ int build_tables(ctr_batch& B) {
     B.bpTotal[c].assign(S, 0);
   }
   B.winZ = 0;
+  B.winDNV = 0;
   for (int i = 0; i < n; ++i) {
     const int c = scan_zg(B, *B.scans[i]->P);
     B.clsScans[c].push_back(i);
-    if (c == 0) B.winZ = std::max(B.winZ, B.scans[i]->P->d.ZN);
+    if (c == 0) {
+      B.winZ = std::max(B.winZ, B.scans[i]->P->d.ZN);
+      B.winDNV = std::max(B.winDNV, B.scans[i]->P->d.DNV);
+    }
   }
   auto up = [&](Mem& m, const void* src, size_t bytes) -> int {
     if (m.p) {
@@ -197,8 +202,8 @@ int step_bp(ctr_batch& B, int s, bool tuning) {
     if (!m) continue;
     const Mem& mem = tuning ? B.tuneBpItems[cz] : B.bpItems[cz];
     const BpBatchItem<T>* items = (const BpBatchItem<T>*)mem.p + (size_t)s * m;
-    int rc = cz == 0 ? launch_bp_batch_win<T>(items, (int)m, B.bpTotal[0][s], B.winZ, B.bpGrid,
-                                              B.bpStrided, B.st)
+    int rc = cz == 0 ? launch_bp_batch_win<T>(items, (int)m, B.bpTotal[0][s], B.winZ,
+                                              B.winDNV, B.bpGrid, B.bpStrided, B.st)
                      : launch_bp_batch<T>(items, (int)m, B.bpTotal[cz][s], cz, B.bpGrid,
                                           B.bpStrided, B.st);
     if (rc) return rc;

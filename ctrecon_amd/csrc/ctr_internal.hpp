@@ -24,9 +24,14 @@ constexpr int kWinMaxZ = 2048;
 //   wx            1 when |src.x| > |src.y| (FP64 decision): the reference back
 //                 projector then uses the x component as the cell width direction
 //                 (DD3_GPU.cu:1453-1460)
+//   azi, azf      az split into the nearest integer and the rest (|azf| <= 1/2): the
+//                 forward projector and its transpose place z footprints relative to
+//                 azi, where FP32 keeps 16x or more finer positions than at az
 struct ViewP {
   float sx, sy, sz, az;
-  int xmarch, bdir, wx, pad1;
+  int xmarch, bdir, wx, azi;
+  float azf;
+  int pad0, pad1, pad2;
 };
 
 // Constants of the reference back projector (DD3_GPU.cu:1428-1437, calDetParas
@@ -74,6 +79,7 @@ struct FpArgs {
   TO* out;
   const ViewP* views;
   const float2* bnd;
+  const double2* bnd64;
   const float4* cols;
   const float* Bz;
   const float* h2;
@@ -133,6 +139,7 @@ struct Plan {
   ViewP* views = nullptr;      // [PN]
   double2* src64 = nullptr;    // [PN]         rotated source (x, y) in FP64
   float2* bnd = nullptr;       // [PN][DNU+1]  (a_b, f_b): t_b(ii) = a_b + ii*f_b
+  double2* bnd64 = nullptr;    // [PN][DNU+1]  the same in FP64 (forward projector, transpose)
   float4* cols = nullptr;      // [PN][DNU]    (r0, r1, G, Q)
   float* Bz = nullptr;         // [DNV+1]      (bz_v - z0)/dz
   float* h2 = nullptr;         // [DNV]        (zds_v - z0)^2
@@ -230,10 +237,11 @@ int launch_fp_batch(const FpBatchItem<T>* items, int nscan, int total, bool colw
 template <typename T>
 int launch_bp_batch(const BpBatchItem<T>* items, int nscan, int total, int zg, int grid,
                     int strided, hipStream_t st);
-// windowed form: one work item per block of 4 columns over all z; zmax = tallest column
+// windowed form: one work item per block of 4 columns over all z; zmax = tallest column,
+// maxDNV = most detector rows of a scan in the list
 template <typename T>
-int launch_bp_batch_win(const BpBatchItem<T>* items, int nscan, int total, int zmax, int grid,
-                        int strided, hipStream_t st);
+int launch_bp_batch_win(const BpBatchItem<T>* items, int nscan, int total, int zmax,
+                        int maxDNV, int grid, int strided, hipStream_t st);
 int launch_reduce_batch(const ReduceItem* items, int nscan, hipStream_t st);
 int launch_to_bf16(uint16_t* dst, const float* src, size_t n, hipStream_t st);
 // the solvers' dithered BF16 shadow of an iterate (to_bf16_dither)

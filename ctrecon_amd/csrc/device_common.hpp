@@ -93,6 +93,11 @@ struct Buf<float> {
   __device__ __forceinline__ float operator[](int i) const {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, i * 4, 0, 0));
   }
+  // element i + j, j wave-uniform: its byte offset goes in the loads' scalar offset, which
+  // the range check does not include, so i + j must lie inside the array
+  __device__ __forceinline__ float at(int i, int j) const {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, i * 4, j * 4, 0));
+  }
   __device__ __forceinline__ void tap3(int o, int t1, int t2, float& a0, float& a1,
                                        float& a2) const {
     a0 = (*this)[o];
@@ -108,6 +113,10 @@ struct Buf<uint16_t> {
                                             0x00020000)) {}
   __device__ __forceinline__ float operator[](int i) const {
     return __uint_as_float((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, i * 2, 0, 0) << 16);
+  }
+  __device__ __forceinline__ float at(int i, int j) const {
+    return __uint_as_float((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, i * 2, j * 2, 0)
+                           << 16);
   }
   // 16-bit loads (two aligned 32-bit loads with half-word selects measured slower: the
   // projectors are VALU-bound)

@@ -58,40 +58,66 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* slot) {
 // cell-centre index units; its mean over the footprint, summed over slices and
 // scaled by dx/|dir| (K), is the ray sum.
 // FUSED: out = (b - A x) * invRow (os_sart.m:24,30-31; b indexed by global view,
-// invRow by the launch's compact view index, NULL for 1) and per-block sums of
-// (b - A x)^2 and b^2.
+// invRow by the launch's compact view index, NULL for 1; out NULL: not written) and
+// per-block sums of (b - A x)^2 and b^2.
 // T: storage type of vol and out (float, or uint16_t for BF16); b and invRow are FP32
 // and the ray sum is accumulated in FP32.
 // ---------------------------------------------------------------------------
-// NJ in-plane cells x 3 z taps of a slice footprint (cell stride cS, mask stride mC):
-// all 3 NJ loads are issued before the first is used, then the taps are combined in the
-// order of the general loop in fp_body.
-// NT = 2 when no lane needs a third z cell (wz2 == 0 everywhere: the term it drops is an
-// exact zero).
+// NJ in-plane cells x 2 or 3 z taps of a slice footprint, all loads issued before the
+// first is used: the cells' weights times their mask bytes already taken (wy0, wy1, wy2)
+// and the z taps read as the fixed run o, o + 1, o + 2 of cell ij (o = ob + ij * cS + k0;
+// one offset per cell, the steps in the loads' immediate offsets, all range-checked: a tap
+// of zero weight reads a neighbouring element of the array, finite, or past its end 0, and
+// adds an exact zero).  NT = 2 when no lane needs a third z cell (the term it drops is an
+// exact zero).  The combining order of the general loop in fp_body.
 template <int NJ, int NT, typename TV>
-__device__ __forceinline__ float fp_cells(const Buf<TV>& V, const CTR_GLOBAL float* maskf, int o,
-                                          int mo, int cS, int mC, int t1, int t2, float wz0,
-                                          float wz1, float wz2, float wj0, float wjl) {
+__device__ __forceinline__ float fp_cells_w(const Buf<TV>& V, int ob, int cS, int k0,
+                                            float wz0, float wz1, float wz2, float wy0,
+                                            float wy1, float wy2) {
   float a[NJ][3];
 #pragma unroll
   for (int ij = 0; ij < NJ; ++ij) {
-    if constexpr (NT == 3) {
-      V.tap3(o + ij * cS, t1, t2, a[ij][0], a[ij][1], a[ij][2]);
-    } else {
-      a[ij][0] = V[o + ij * cS];
-      a[ij][1] = V[o + ij * cS + t1];
-    }
+    const int o = k0 + (ob + ij * cS);
+    a[ij][0] = V[o];
+    a[ij][1] = V[o + 1];
+    if constexpr (NT == 3) a[ij][2] = V[o + 2];
   }
   float s = 0.f;
 #pragma unroll
   for (int ij = 0; ij < NJ; ++ij) {
-    const float wy = ((ij == 0) ? wj0 : (ij == NJ - 1 ? wjl : 1.f)) * maskf[mo + ij * mC];
+    const float wy = ij == 0 ? wy0 : (ij == 1 ? wy1 : wy2);
     float g = wz0 * a[ij][0];
     g = fmaf(wz1, a[ij][1], g);
     if constexpr (NT == 3) g = fmaf(wz2, a[ij][2], g);
     s = fmaf(wy, g, s);
   }
   return s;
+}
+
+// In-plane part of slice ii of one detector column's footprint: the boundary lines
+// t_b(ii) = a_b + ii f_b of its two boundaries (FP64: positions run to the slice width,
+// where FP32 keeps only ~3e-5 of a cell), the footprint [ylo, yhi] clipped to the mask's
+// extent e on the slice, its cells j0 .. j0 + nj - 1 with first / last overlap weights
+// (FP64 differences, rounded once), the z magnification r and 1 / ((yhi - ylo) r).
+// False when the clipped footprint is empty.
+struct SliceIP {
+  float r, inv, wj0, wjl;
+  int j0, nj;
+};
+__device__ __forceinline__ bool fp_inplane(const double2& bl, const double2& br, const float4& ct,
+                                           const float2& e, int ii, int NI, SliceIP& o) {
+  const double fi = (double)ii;
+  const double tl = fma(fi, bl.y, bl.x), tr = fma(fi, br.y, br.x);
+  const double ylo = fmin(tl, tr), yhi = fmax(tl, tr);
+  const double clo = fmax(ylo, (double)e.x), chi = fmin(yhi, (double)e.y);
+  if (!(clo < chi)) return false;
+  o.r = fmaf((float)ii, ct.y, ct.x);
+  o.inv = __builtin_amdgcn_rcpf((float)(yhi - ylo) * o.r);
+  o.j0 = (int)floor(clo + 0.5);
+  o.nj = min((int)floor(chi + 0.5), NI - 1) - o.j0 + 1;
+  o.wj0 = (float)(o.nj == 1 ? chi - clo : ((double)o.j0 + 0.5) - clo);
+  o.wjl = (float)(chi - ((double)(o.j0 + o.nj - 1) - 0.5));
+  return true;
 }
 
 // ONES: the projection of a volume of ones (the row sums of OS-SART), read from no
@@ -111,8 +137,8 @@ __device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk
     const int u = COLW ? (ublk * kWaves + wid) : ray / d.DNV;
     const int v = ray - u * d.DNV;
     const ViewP vp = gload(A.views, p);
-    const float2 bl = gload(A.bnd, (size_t)p * (d.DNU + 1) + u);
-    const float2 br = gload(A.bnd, (size_t)p * (d.DNU + 1) + u + 1);
+    const double2 bl = gload(A.bnd64, (size_t)p * (d.DNU + 1) + u);
+    const double2 br = gload(A.bnd64, (size_t)p * (d.DNU + 1) + u + 1);
     const float4 ct = gload(A.cols, (size_t)p * d.DNU + u);
     const float Bv = gp(A.Bz)[v], Bv1 = gp(A.Bz)[v + 1];
     int NS, NI, sS, cS, mS, mC;
@@ -121,17 +147,19 @@ __device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk
     } else {
       NS = d.YN; NI = d.XN; sS = d.XN * d.ZN; cS = d.ZN; mS = d.XN; mC = 1;
     }
-    const float ZNh = (float)d.ZN - 0.5f;
+    // z in cell-index units relative to azi (exact shifts of the volume's bounds)
+    const float zlo_b = -0.5f - (float)vp.azi, zhi_b = ((float)d.ZN - 0.5f) - (float)vp.azi;
     const int ZN1 = d.ZN - 1;
     const float2* me = A.mext + (vp.xmarch ? 0 : d.XN);
-    const Buf<TV> V(A.vol, (uint32_t)(d.XN * d.YN * d.ZN));
+    const int NV = d.XN * d.YN * d.ZN;
+    const Buf<TV> V(A.vol, (uint32_t)NV);
     const CTR_GLOBAL float* maskf = gp(A.maskf);
     float acc = 0.f;
     // one slice: footprint [clo, chi] x [zl, zh] over the cells j0 .. j0 + nj - 1
     auto slice = [&](int ii, float r, float inv, int j0, int nj, float wj0, float wjl) {
-      float zl = fmaf(r, Bv, vp.az), zh = fmaf(r, Bv1, vp.az);
-      zl = fmaxf(zl, -0.5f);
-      zh = fminf(zh, ZNh);
+      float zl = fmaf(r, Bv, vp.azf), zh = fmaf(r, Bv1, vp.azf);
+      zl = fmaxf(zl, zlo_b);
+      zh = fminf(zh, zhi_b);
       const bool zok = zl < zh;
       if (!__any(zok)) return;
       if constexpr (ONES) {
@@ -146,7 +174,7 @@ __device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk
       // of cells k0, k0 + 1, k0 + 2 are min(zh, e) - zl, clamp(dk, 0, 1), clamp(dk - 1, 0, 1)
       const float fk = floorf(zl + 0.5f);
       const float ek = fk + 0.5f;
-      const int k0 = min((int)fk, ZN1);
+      const int k0 = min((int)fk + vp.azi, ZN1);
       const float dk = zok ? zh - ek : -1.f;
       const float wz0 = zok ? fminf(zh, ek) - zl : 0.f;
       const float wz1 = __builtin_amdgcn_fmed3f(dk, 0.f, 1.f);
@@ -157,37 +185,19 @@ __device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk
       const int t1 = dk > 0.f ? 1 : 0, t2 = dk > 1.f ? 2 : 0;
       float s = 0.f;
       if (!__any(dk > 2.f)) {
-        if (COLW && sizeof(TV) == 4 && nj <= 3) {
-          // nj is uniform here: straight-line code per cell count, all loads first (FP32
-          // volumes; with BF16 volumes the sub-dword gathers run faster one cell at a time)
-          if (!__any(dk > 1.f)) {  // at most two z cells in every lane's footprint
-            switch (nj) {
-              case 1: s = fp_cells<1, 2>(V, maskf, o, mo, cS, mC, t1, t2, wz0, wz1, wz2, wj0, wjl); break;
-              case 2: s = fp_cells<2, 2>(V, maskf, o, mo, cS, mC, t1, t2, wz0, wz1, wz2, wj0, wjl); break;
-              default: s = fp_cells<3, 2>(V, maskf, o, mo, cS, mC, t1, t2, wz0, wz1, wz2, wj0, wjl); break;
-            }
-          } else {
-            switch (nj) {
-              case 1: s = fp_cells<1, 3>(V, maskf, o, mo, cS, mC, t1, t2, wz0, wz1, wz2, wj0, wjl); break;
-              case 2: s = fp_cells<2, 3>(V, maskf, o, mo, cS, mC, t1, t2, wz0, wz1, wz2, wj0, wjl); break;
-              default: s = fp_cells<3, 3>(V, maskf, o, mo, cS, mC, t1, t2, wz0, wz1, wz2, wj0, wjl); break;
-            }
-          }
-        } else {
-          for (int ij = 0; ij < nj; ++ij) {
-            const float wy = ((ij == 0) ? wj0 : (ij == nj - 1 ? wjl : 1.f)) * maskf[mo];
-            float g = wz0 * V[o];
-            g = fmaf(wz1, V[o + t1], g);
-            g = fmaf(wz2, V[o + t2], g);
-            s = fmaf(wy, g, s);
-            o += cS;
-            mo += mC;
-          }
+        for (int ij = 0; ij < nj; ++ij) {
+          const float wy = ((ij == 0) ? wj0 : (ij == nj - 1 ? wjl : 1.f)) * maskf[mo];
+          float g = wz0 * V[o];
+          g = fmaf(wz1, V[o + t1], g);
+          g = fmaf(wz2, V[o + t2], g);
+          s = fmaf(wy, g, s);
+          o += cS;
+          mo += mC;
         }
       } else {
-        const int nk = zok ? min((int)floorf(zh + 0.5f), ZN1) - k0 + 1 : 0;
+        const int nk = zok ? min((int)floorf(zh + 0.5f) + vp.azi, ZN1) - k0 + 1 : 0;
         float wf, wzl;
-        edge_weights(zl, zh, k0, nk, wf, wzl);
+        edge_weights(zl, zh, k0 - vp.azi, nk, wf, wzl);
         for (int ij = 0; ij < nj; ++ij) {
           const float wy = ((ij == 0) ? wj0 : (ij == nj - 1 ? wjl : 1.f)) * maskf[mo];
           float g = 0.f;
@@ -200,33 +210,138 @@ __device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk
       }
       acc = fmaf(s, inv, acc);
     };
-    if constexpr (COLW) {
+    if constexpr (COLW && !ONES) {
       // the in-plane part of a slice is uniform over the wave: lane l evaluates it for
-      // slice ii0 + l, and the loop over the slices the footprint meets reads it back
-      // with readlane into scalar registers (same arithmetic per slice)
+      // slice ii0 + l (footprint, 1 / its area, and each cell's edge weight times its mask
+      // byte) into the wave's LDS table, and the loop over the slices the footprint meets
+      // reads a slice's entry at a uniform address (same arithmetic per slice)
+      __shared__ float4 s_slices[kWaves * 128];
+      float4* rec = s_slices + wid * 128;
+      const int lane = threadIdx.x & 63;
+      for (int ii0 = 0; ii0 < NS; ii0 += 64) {
+        const int il = ii0 + lane;
+        float q_r = 0.f, q_inv = 0.f, q_wj0 = 0.f, q_wjl = 0.f, wy0 = 0.f, wy1 = 0.f, wy2 = 0.f;
+        int q_j0 = 0, q_nj = 0;
+        bool ok = false;
+        SliceIP si;
+        // clipped to the mask's extent on the slice (inside [-0.5, NI - 0.5]): cells
+        // outside it are zero
+        if (il < NS && fp_inplane(bl, br, ct, gload(me, il), il, NI, si)) {
+          ok = true;
+          {
+            q_r = si.r;
+            q_inv = si.inv;
+            q_j0 = si.j0;
+            q_nj = si.nj;
+            q_wj0 = si.wj0;
+            q_wjl = si.wjl;
+            const int mo = il * mS + q_j0 * mC;
+            wy0 = q_wj0 * maskf[mo];
+            if (q_nj >= 2) wy1 = (q_nj == 2 ? q_wjl : 1.f) * maskf[mo + mC];
+            if (q_nj >= 3) wy2 = q_wjl * maskf[mo + 2 * mC];
+          }
+        }
+        rec[2 * lane] = make_float4(q_r, q_inv, wy0, wy1);
+        rec[2 * lane + 1] = make_float4(wy2, q_wj0, q_wjl, 0.f);
+        uint64_t live = __ballot(ok);
+        while (live) {
+          const int j = __builtin_ctzll(live);
+          live &= live - 1;
+          const int ii = ii0 + j;
+          const int j0 = __builtin_amdgcn_readlane(q_j0, j);
+          const int nj = __builtin_amdgcn_readlane(q_nj, j);
+          const float4 ra = rec[2 * j], rb = rec[2 * j + 1];
+          const float r = ra.x;
+          float zl = fmaf(r, Bv, vp.azf), zh = fmaf(r, Bv1, vp.azf);
+          zl = fmaxf(zl, zlo_b);
+          zh = __builtin_elementwise_minimum(zh, zhi_b);
+          const bool zok = zl < zh;
+          if (!__any(zok)) continue;
+          // first cell k0 = floor(zl + .5) (+ azi) and its upper edge e; with dk = zh - e
+          // the overlaps of cells k0, k0 + 1, k0 + 2 are min(zh, e) - zl, clamp(dk, 0, 1),
+          // clamp(dk - 1, 0, 1)
+          const float fk = floorf(zl + 0.5f);
+          const float ek = fk + 0.5f;
+          const int k0 = min((int)fk + vp.azi, ZN1);
+          // outside [-0.5, ZN - 0.5] (zl >= zh): dk < 0 and wz0 <= 0 clamped, so every
+          // weight is zero
+          const float dk = zh - ek;
+          const float wz0 = fmaxf(__builtin_elementwise_minimum(zh, ek) - zl, 0.f);
+          const float wz1 = __builtin_amdgcn_fmed3f(dk, 0.f, 1.f);
+          const float wz2 = __builtin_amdgcn_fmed3f(dk - 1.f, 0.f, 1.f);
+          const int ob = ii * sS + j0 * cS;  // wave-uniform
+          float s = 0.f;
+          if (!__any(dk > 2.f)) {
+            // the fixed runs of fp_cells_w stay inside the array (its merged multi-element
+            // loads never straddle the end): not on the array's last column
+            if (sizeof(TV) == 4 && nj <= 3 && ob + (nj - 1) * cS + d.ZN + 2 <= NV) {
+              // nj is uniform: straight-line code per cell count, all loads first (FP32
+              // volumes; with BF16 volumes the sub-dword gathers run faster one cell at
+              // a time)
+              if (!__any(dk > 1.f)) {  // at most two z cells in every lane's footprint
+                switch (nj) {
+                  case 1: s = fp_cells_w<1, 2>(V, ob, cS, k0, wz0, wz1, wz2, ra.z, ra.w, rb.x); break;
+                  case 2: s = fp_cells_w<2, 2>(V, ob, cS, k0, wz0, wz1, wz2, ra.z, ra.w, rb.x); break;
+                  default: s = fp_cells_w<3, 2>(V, ob, cS, k0, wz0, wz1, wz2, ra.z, ra.w, rb.x); break;
+                }
+              } else {
+                switch (nj) {
+                  case 1: s = fp_cells_w<1, 3>(V, ob, cS, k0, wz0, wz1, wz2, ra.z, ra.w, rb.x); break;
+                  case 2: s = fp_cells_w<2, 3>(V, ob, cS, k0, wz0, wz1, wz2, ra.z, ra.w, rb.x); break;
+                  default: s = fp_cells_w<3, 3>(V, ob, cS, k0, wz0, wz1, wz2, ra.z, ra.w, rb.x); break;
+                }
+              }
+            } else {
+              // taps of zero weight re-read the first cell
+              const int t1 = dk > 0.f ? 1 : 0, t2 = dk > 1.f ? 2 : 0;
+              int mo = ii * mS + j0 * mC;
+              int o = ob + k0;
+              for (int ij = 0; ij < nj; ++ij) {
+                const float wy = ((ij == 0) ? rb.y : (ij == nj - 1 ? rb.z : 1.f)) * maskf[mo];
+                float g = wz0 * V[o];
+                g = fmaf(wz1, V[o + t1], g);
+                g = fmaf(wz2, V[o + t2], g);
+                s = fmaf(wy, g, s);
+                o += cS;
+                mo += mC;
+              }
+            }
+          } else {
+            const int nk = zok ? min((int)floorf(zh + 0.5f) + vp.azi, ZN1) - k0 + 1 : 0;
+            float wf, wzl;
+            edge_weights(zl, zh, k0 - vp.azi, nk, wf, wzl);
+            int mo = ii * mS + j0 * mC;
+            int o = ob + k0;
+            for (int ij = 0; ij < nj; ++ij) {
+              const float wy = ((ij == 0) ? rb.y : (ij == nj - 1 ? rb.z : 1.f)) * maskf[mo];
+              float g = 0.f;
+              for (int ik = 0; ik < nk; ++ik)
+                g = fmaf((ik == 0) ? wz0 : (ik == nk - 1 ? wzl : 1.f), V[o + ik], g);
+              s = fmaf(wy, g, s);
+              o += cS;
+              mo += mC;
+            }
+          }
+          acc = fmaf(s, ra.y, acc);
+        }
+      }
+    } else if constexpr (COLW) {
+      // ONES: lane l evaluates the in-plane part of slice ii0 + l, read back with readlane
       const int lane = threadIdx.x & 63;
       for (int ii0 = 0; ii0 < NS; ii0 += 64) {
         const int il = ii0 + lane;
         float q_r = 0.f, q_inv = 0.f, q_wj0 = 0.f, q_wjl = 0.f;
         int q_j0 = 0, q_nj = 0;
         bool ok = false;
-        if (il < NS) {
-          const float fi = (float)il;
-          const float tl = fmaf(fi, bl.y, bl.x);
-          const float tr = fmaf(fi, br.y, br.x);
-          const float ylo = fminf(tl, tr), yhi = fmaxf(tl, tr);
-          // clip to the mask's extent on this slice (inside [-0.5, NI - 0.5]); cells
-          // outside it are zero
-          const float2 e = gload(me, il);
-          const float clo = fmaxf(ylo, e.x), chi = fminf(yhi, e.y);
-          ok = clo < chi;
-          if (ok) {
-            q_r = fmaf(fi, ct.y, ct.x);
-            q_inv = __builtin_amdgcn_rcpf((yhi - ylo) * q_r);
-            q_j0 = (int)floorf(clo + 0.5f);
-            q_nj = min((int)floorf(chi + 0.5f), NI - 1) - q_j0 + 1;
-            edge_weights(clo, chi, q_j0, q_nj, q_wj0, q_wjl);
-          }
+        SliceIP si;
+        if (il < NS && fp_inplane(bl, br, ct, gload(me, il), il, NI, si)) {
+          ok = true;
+          q_r = si.r;
+          q_inv = si.inv;
+          q_j0 = si.j0;
+          q_nj = si.nj;
+          q_wj0 = si.wj0;
+          q_wjl = si.wjl;
         }
         uint64_t live = __ballot(ok);
         while (live) {
@@ -238,20 +353,9 @@ __device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk
       }
     } else {
       for (int ii = 0; ii < NS; ++ii) {
-        const float fi = (float)ii;
-        const float tl = fmaf(fi, bl.y, bl.x);
-        const float tr = fmaf(fi, br.y, br.x);
-        const float ylo = fminf(tl, tr), yhi = fmaxf(tl, tr);
-        const float2 e = gload(me, ii);
-        const float clo = fmaxf(ylo, e.x), chi = fminf(yhi, e.y);
-        if (!(clo < chi)) continue;
-        const float r = fmaf(fi, ct.y, ct.x);
-        const float inv = __builtin_amdgcn_rcpf((yhi - ylo) * r);
-        const int j0 = (int)floorf(clo + 0.5f);
-        const int nj = min((int)floorf(chi + 0.5f), NI - 1) - j0 + 1;
-        float wj0, wjl;
-        edge_weights(clo, chi, j0, nj, wj0, wjl);
-        slice(ii, r, inv, j0, nj, wj0, wjl);
+        SliceIP si;
+        if (fp_inplane(bl, br, ct, gload(me, ii), ii, NI, si))
+          slice(ii, si.r, si.inv, si.j0, si.nj, si.wj0, si.wjl);
       }
     }
     const float K = ct.w * __builtin_amdgcn_sqrtf(ct.z + gp(A.h2)[v]) * gp(A.invdB)[v];
@@ -261,7 +365,7 @@ __device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk
       const size_t og = ((size_t)p * d.DNU + u) * d.DNV + v;
       const float bb = gp(A.b)[og];
       const float diff = bb - val;
-      st(gp(A.out), o, A.invRow ? diff * gp(A.invRow)[o] : diff);
+      if (A.out) st(gp(A.out), o, A.invRow ? diff * gp(A.invRow)[o] : diff);
       res = (double)diff * (double)diff;
       bsq = (double)bb * (double)bb;
     } else {
@@ -519,26 +623,33 @@ __device__ __forceinline__ float bp_cell_sum(const Buf<T>& pp, int u0, int nu, f
   return s;
 }
 
-// Rows per wave staged for the windowed back projector (DNV <= kStageRows), plus two
-// rows of zeros read by taps past the last row.
+// Staged windowed back projector (DNV <= kStageRows): per wave, the current view's
+// combined rows (kStagePad floats: two rows of zeros past the last row for taps there),
+// then a table of 64 view records (kRecF floats each: row-shadow constants, cosVal
+// constants and the four column weights, written by lane l for view k0 + l).  kStageF
+// floats per wave, a multiple of 4 so every wave's table is 16-byte aligned.
 constexpr int kStageRows = 64;
 constexpr int kStagePad = kStageRows + 2;
-constexpr int kStageViews = 4;  // views staged together
+constexpr int kRowsF = kStagePad + 2;  // rows, then the record table 16-byte aligned
+constexpr int kRecF = 12;
+constexpr int kStageF = kRowsF + 64 * kRecF;
+static_assert(kRowsF % 4 == 0 && kStageF % 4 == 0, "per-wave stage: 16-byte multiples");
 
 // bp_cell_sum over a view whose columns are already combined per row in LDS
-// (lR[v] = sum_u wu y[u][v], rows past DNV zero): the same weights, chosen by the same
-// test (nu4: the view has at most four columns), and the same sum (taps past the shadow
-// add exact zeros, as the re-read taps do there).
-__device__ __forceinline__ float bp_lds_sum(const float* lR, float vlo, float vhi, int DNV,
-                                            bool nu4) {
+// (lR[v] = sum_u wu y[u][v], rows past DNV zero), with the same sum: the row weights
+// min(vhi, e) - vlo, clamp(d, 0, 1), clamp(d - 1, 0, 1) of a shadow over at most three
+// rows are exactly the first / interior / last weights of edge_weights (d = vhi - e and
+// d - 1 are exact differences), and taps past the shadow add exact zeros, so the choice
+// between the two forms may be made for the whole wave.
+__device__ __forceinline__ float bp_lds_sum(const float* lR, float vlo, float vhi, int DNV) {
 #pragma clang fp contract(off)
   const float fv = floorf(vlo + 0.5f);
   const float e = fv + 0.5f;
   const int v0 = (int)fv;
   const float d = vhi - e;
   float s;
-  if (nu4 && d <= 2.f) {
-    const float wv0 = fminf(vhi, e) - vlo;
+  if (!__any(d > 2.f)) {
+    const float wv0 = __builtin_elementwise_minimum(vhi, e) - vlo;
     const float wv1 = __builtin_amdgcn_fmed3f(d, 0.f, 1.f);
     s = wv0 * lR[v0];
     s = fmaf(wv1, lR[v0 + 1], s);
@@ -658,13 +769,16 @@ __device__ __forceinline__ void bp_ref_body(const BpArgs<T>& A, int col, int zbl
 // the column weights to Wu = wu0 + (nu - 2) + fh (wu0 alone for one column), so a voxel
 // adds cosVal * Wu * (vhi - vlo) (the same sum in another rounding order).
 //
-// lR (kStageViews * kStagePad floats of this wave, or NULL): for BF16 sinograms with
-// DNV <= kStageRows, each visible view's columns are combined per row once (R[v] =
-// sum_u wu y[u][v] in lane v, four views' loads issued together) and the voxels read R
-// from LDS; the sums are those of the direct gathers, bit for bit.
+// stg (kStageF floats of this wave, 16-byte aligned, or NULL): with DNV <= kStageRows the
+// views are staged.  Lane l writes view k0 + l's record (row-shadow and cosVal constants,
+// column weights) to the wave's table; then, visible view by visible view, every lane
+// loads its row of the view's first four columns (issued while the previous view's voxels
+// are summed), combines them (R[v] = sum_u wu y[u][v], the column order and rounding of
+// bp_cells) into LDS, and the voxels of the window read two or three values of R there;
+// the sums are those of the direct gathers, bit for bit.
 template <bool UPDATE, typename T, bool ONES = false>
 __device__ __forceinline__ void bp_win_body(const BpArgs<T>& A, int col, float* lacc,
-                                            float* lR) {
+                                            float* stg) {
   const Dims& d = A.d;
   const BpConsts64& c64 = A.c64;
   const int lane = threadIdx.x & 63;
@@ -686,12 +800,11 @@ __device__ __forceinline__ void bp_win_body(const BpArgs<T>& A, int col, float* 
   const float idz = 1.f / d.dz;
   const int DNV = d.DNV;
   const uint32_t frame = (uint32_t)(d.DNU * d.DNV);
-  // staged for BF16 sinograms (each 16-bit value widened once per view instead of once per
-  // tap: 6.7 vs 7.5 ms per clinical subset); FP32 ones gather directly (6.5 ms staged or not)
-  constexpr bool kStageT = !ONES && sizeof(T) == 2;
-  const bool stage = kStageT && lR != nullptr && DNV <= kStageRows;
-  if (stage && lane < kStageViews * (kStagePad - kStageRows))
-    lR[(lane >> 1) * kStagePad + kStageRows + (lane & 1)] = 0.f;
+  const bool stage = !ONES && stg != nullptr && DNV <= kStageRows;
+  float* lR = stg;                                         // rows of the current view
+  float4* rec = reinterpret_cast<float4*>(stg + kRowsF);  // [64][3] float4 records
+  if (stage && lane < kStagePad - kStageRows) lR[kStageRows + lane] = 0.f;
+  const int row = min(lane, DNV - 1);  // the row a lane stages
   for (int k0 = 0; k0 < A.vc; k0 += 64) {
     const int kl = k0 + lane;
     BpView q{};
@@ -708,74 +821,64 @@ __device__ __forceinline__ void bp_win_body(const BpArgs<T>& A, int col, float* 
       q_z1 = min((int)ceilf(zb + 1e-2f) - 1, d.ZN - 1);
     }
     uint64_t live = __ballot(q.uvis && q_z0 <= q_z1);
-    while (stage && live) {
-      // up to kStageViews visible views at a time: every lane loads its row of their
-      // first four columns (all loads issued together: out-of-range columns / rows are
-      // clamped and weighted 0), combines them (R[v] = sum_u wu y[u][v], the column order
-      // and rounding of bp_cells) and stores them; then each view's window reads them
+    if (stage && live) {
 #pragma clang fp contract(off)
-      int jv[kStageViews];
-      int ng = 0;
-#pragma unroll
-      for (int g = 0; g < kStageViews; ++g) {
-        jv[g] = live ? __builtin_ctzll(live) : 0;
-        if (live) {
-          live &= live - 1;
-          ng = g + 1;
-        }
+      {
+        // column weights of view k0 + lane: first wu0, interior 1, last fh, past nu 0
+        const int nu = q.nu;
+        const float w1 = nu > 1 ? (nu == 2 ? q.fh : 1.f) : 0.f;
+        const float w2 = nu > 2 ? (nu == 3 ? q.fh : 1.f) : 0.f;
+        const float w3 = nu > 3 ? (nu == 4 ? q.fh : 1.f) : 0.f;
+        rec[3 * lane + 0] = make_float4(q.rlS, q.cA, q.cB, q.sz);
+        rec[3 * lane + 1] = make_float4(q.l2, q.cws, q.wu0, w1);
+        rec[3 * lane + 2] = make_float4(w2, w3, 0.f, 0.f);
       }
-      float yv[kStageViews][4];
+      // rows of view k0 + j: columns u0 + min(iu, nu - 1) (a repeated column has weight 0)
+      auto load_rows = [&](int j, float (&yv)[4]) {
+        const int un = __builtin_amdgcn_readlane(q.u0, j);
+        const int nn = __builtin_amdgcn_readlane(q.nu, j);
+        const Buf<T> pn(A.prj + (size_t)(k0 + j) * frame, frame);
 #pragma unroll
-      for (int g = 0; g < kStageViews; ++g) {
-        const int jg = g < ng ? jv[g] : jv[0];
-        const int un = __builtin_amdgcn_readlane(q.u0, jg);
-        const int nn = __builtin_amdgcn_readlane(q.nu, jg);
-        const Buf<T> pn(A.prj + (size_t)(k0 + jg) * frame, frame);
-        const int row = min(lane, DNV - 1);
-#pragma unroll
-        for (int iu = 0; iu < 4; ++iu) yv[g][iu] = pn[(un + min(iu, nn - 1)) * DNV + row];
-      }
-      __builtin_amdgcn_wave_barrier();
-#pragma unroll
-      for (int g = 0; g < kStageViews; ++g) {
-        if (g >= ng) break;
-        const int jg = jv[g];
-        const int un = __builtin_amdgcn_readlane(q.u0, jg);
-        const int nn = __builtin_amdgcn_readlane(q.nu, jg);
-        const float wu0 = rdl(q.wu0, jg), fh = rdl(q.fh, jg);
-        float R = wu0 * yv[g][0];
-#pragma unroll
-        for (int iu = 1; iu < 4; ++iu)
-          R = fmaf(iu < nn ? (iu == nn - 1 ? fh : 1.f) : 0.f, yv[g][iu], R);
-        if (nn > 4) {
-          const Buf<T> pn(A.prj + (size_t)(k0 + jg) * frame, frame);
-          for (int iu = 4; iu < nn; ++iu)
-            R = fmaf((iu == nn - 1) ? fh : 1.f, pn[(un + iu) * DNV + min(lane, DNV - 1)], R);
-        }
-        lR[g * kStagePad + lane] = lane < DNV ? R : 0.f;
-      }
-      __builtin_amdgcn_wave_barrier();
-#pragma unroll
-      for (int g = 0; g < kStageViews; ++g) {
-        if (g >= ng) break;
-        const int j = jv[g];
-        const float rlS = rdl(q.rlS, j), cA = rdl(q.cA, j), cB = rdl(q.cB, j);
-        const float vsz = rdl(q.sz, j), l2f = rdl(q.l2, j), cws = rdl(q.cws, j);
+        for (int iu = 0; iu < 4; ++iu) yv[iu] = pn.at(row, (un + min(iu, nn - 1)) * DNV);
+      };
+      int j = __builtin_ctzll(live);
+      float yv[4];
+      load_rows(j, yv);
+      while (true) {
         const int nu = __builtin_amdgcn_readlane(q.nu, j);
         const int z0 = __builtin_amdgcn_readlane(q_z0, j), z1 = __builtin_amdgcn_readlane(q_z1, j);
-        const float* lRg = lR + g * kStagePad;
+        const float4 r0 = rec[3 * j], r1 = rec[3 * j + 1], r2 = rec[3 * j + 2];
+        float R = r1.z * yv[0];
+        R = fmaf(r1.w, yv[1], R);
+        R = fmaf(r2.x, yv[2], R);
+        R = fmaf(r2.y, yv[3], R);
+        if (nu > 4) {
+          const int un = __builtin_amdgcn_readlane(q.u0, j);
+          const float fh = rdl(q.fh, j);
+          const Buf<T> pn(A.prj + (size_t)(k0 + j) * frame, frame);
+          for (int iu = 4; iu < nu; ++iu)
+            R = fmaf((iu == nu - 1) ? fh : 1.f, pn.at(row, (un + iu) * DNV), R);
+        }
+        lR[lane] = lane < DNV ? R : 0.f;
+        // the next visible view's rows load while this view's voxels are summed
+        live &= live - 1;
+        const int jn = live ? __builtin_ctzll(live) : -1;
+        if (jn >= 0) load_rows(jn, yv);
+        const float rlS = r0.x, cA = r0.y, cB = r0.z, vsz = r0.w, l2f = r1.x, cws = r1.y;
         for (int zz = z0; zz <= z1; zz += 64) {
           const int z = zz + lane;
           const float dZ = fmaf((float)z - d.ocz, d.dz, -vsz);
           const float vlo = fmaxf(fmaf(dZ, rlS, cA), -0.5f);
-          const float vhi = fminf(fmaf(dZ, rlS, cB), DNVh);
-          const bool vok = z <= z1 && vlo < vhi;
-          if (!vok) continue;
-          const float cosVal = cws * __builtin_amdgcn_sqrtf(fmaf(dZ, dZ, l2f));
-          lacc[z] = fmaf(cosVal, bp_lds_sum(lRg, vlo, vhi, DNV, nu <= 4), lacc[z]);
+          const float vhi = __builtin_elementwise_minimum(fmaf(dZ, rlS, cB), DNVh);
+          if (z <= z1 && vlo < vhi) {
+            const float cosVal = cws * __builtin_amdgcn_sqrtf(fmaf(dZ, dZ, l2f));
+            lacc[z] = fmaf(cosVal, bp_lds_sum(lR, vlo, vhi, DNV), lacc[z]);
+          }
         }
+        if (jn < 0) break;
+        j = jn;
       }
-      __builtin_amdgcn_wave_barrier();
+      continue;
     }
     while (live) {
       const int j = __builtin_ctzll(live);
@@ -812,16 +915,18 @@ __global__ __launch_bounds__(kBlock) void bp_ref_kernel(BpArgs<T> A) {
   bp_ref_body<UPDATE, ZG, T>(A, col, blockIdx.y);
 }
 
-// Windowed reference back projection; dynamic LDS: kWaves * ZN floats of column sums,
-// then (BF16 sinograms) kWaves * kStageViews * kStagePad floats of staged rows.
-// At most 72 VGPRs (7 waves per SIMD; no scratch): the staged BF16 update otherwise
-// takes 77 and runs at 6 waves per SIMD.
+// Windowed reference back projection; dynamic LDS: kWaves * kStageF floats of staged
+// views (DNV <= kStageRows; none otherwise), then kWaves * ZN floats of column sums.
+// At most 72 VGPRs (7 waves per SIMD; no scratch).
 template <bool UPDATE, typename T>
 __global__ __launch_bounds__(kBlock, 7) void bp_win_kernel(BpArgs<T> A) {
-  extern __shared__ float lds_acc[];
+  extern __shared__ float4 lds_win[];
+  float* lds = reinterpret_cast<float*>(lds_win);
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  bp_win_body<UPDATE, T>(A, blockIdx.x * kWaves + wid, lds_acc + wid * A.d.ZN,
-                         lds_acc + kWaves * A.d.ZN + wid * kStageViews * kStagePad);
+  const bool stage = A.d.DNV <= kStageRows;
+  float* lacc = lds + (stage ? kWaves * kStageF : 0) + wid * A.d.ZN;
+  bp_win_body<UPDATE, T>(A, blockIdx.x * kWaves + wid, lacc,
+                         stage ? lds + wid * kStageF : nullptr);
 }
 
 // Back projection of a sinogram of ones (A.prj unused); dynamic LDS as bp_win_kernel.
@@ -833,13 +938,15 @@ __global__ __launch_bounds__(kBlock) void bp_win_ones_kernel(BpArgs<float> A) {
 }
 
 // Batched windowed reference back projection: flat work list of (scan, block of 4
-// columns); dynamic LDS: kWaves * zmax floats (zmax: the tallest column of the batch),
-// then (BF16 sinograms) kWaves * kStageViews * kStagePad floats of staged rows.
+// columns); dynamic LDS: kWaves * kStageF floats of staged views (every scan of the batch
+// has DNV <= kStageRows: staged; otherwise none), then kWaves * zmax floats (zmax: the
+// tallest column of the batch).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void bp_batch_win_kernel(const BpBatchItem<T>* __restrict__ D,
                                                               int nscan, int total, int sched,
-                                                              int zmax) {
-  extern __shared__ float lds_acc[];
+                                                              int zmax, int staged) {
+  extern __shared__ float4 lds_win[];
+  float* lds = reinterpret_cast<float*>(lds_win);
   int i0, step, i1;
   batch_range(D, nscan, total, sched, i0, step, i1);
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -848,8 +955,9 @@ __global__ __launch_bounds__(kBlock) void bp_batch_win_kernel(const BpBatchItem<
     if (sched == 1) s = 0;
     while (s + 1 < nscan && D[s + 1].begin <= item) ++s;
     const BpBatchItem<T>& B = D[s];
-    bp_win_body<true, T>(B.a, (item - B.begin) * kWaves + wid, lds_acc + wid * zmax,
-                         lds_acc + kWaves * zmax + wid * kStageViews * kStagePad);
+    bp_win_body<true, T>(B.a, (item - B.begin) * kWaves + wid,
+                         lds + (staged ? kWaves * kStageF : 0) + wid * zmax,
+                         staged ? lds + wid * kStageF : nullptr);
   }
 }
 
@@ -857,16 +965,18 @@ __global__ __launch_bounds__(kBlock) void bp_batch_win_kernel(const BpBatchItem<
 // loop, so nothing stays live around the column body.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void bp_batch_win1_kernel(
-    const BpBatchItem<T>* __restrict__ D, int nscan, int total, int zmax) {
-  extern __shared__ float lds_acc[];
+    const BpBatchItem<T>* __restrict__ D, int nscan, int total, int zmax, int staged) {
+  extern __shared__ float4 lds_win[];
+  float* lds = reinterpret_cast<float*>(lds_win);
   int i0, step, i1;
   batch_range(D, nscan, total, 2, i0, step, i1);
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   int s = 0;
   while (s + 1 < nscan && D[s + 1].begin <= i0) ++s;
   const BpBatchItem<T>& B = D[s];
-  bp_win_body<true, T>(B.a, (i0 - B.begin) * kWaves + wid, lds_acc + wid * zmax,
-                       lds_acc + kWaves * zmax + wid * kStageViews * kStagePad);
+  bp_win_body<true, T>(B.a, (i0 - B.begin) * kWaves + wid,
+                       lds + (staged ? kWaves * kStageF : 0) + wid * zmax,
+                       staged ? lds + wid * kStageF : nullptr);
 }
 
 // Batched persistent reference back projection: flat work list of (scan, z block,
@@ -893,14 +1003,14 @@ __global__ __launch_bounds__(kBlock) void bp_batch_kernel(const BpBatchItem<T>* 
 // Matched adjoint: the exact transpose of fp_kernel, in gather form.
 // Input yp is the sinogram already multiplied by K * invdB (scale_matched_kernel).
 // For each view the voxel's in-plane cell is located among the column footprints
-// on its slice with the same FP32 boundary expressions fp_kernel evaluates, and
-// its z cell among the row footprints; the overlap weights are those of the
-// forward projector.
+// on its slice with the FP64 boundary lines fp_kernel evaluates, and its z cell
+// among the row footprints (FP32, relative to the view's azi as in fp_kernel); the
+// overlap weights are those of the forward projector.
 // ---------------------------------------------------------------------------
 template <bool UPDATE, int ZG>
 __global__ __launch_bounds__(kBlock) void bp_matched_kernel(
     const float* __restrict__ yp, float* __restrict__ vol, const ViewP* __restrict__ views,
-    const float2* __restrict__ bnd, const float4* __restrict__ cols,
+    const double2* __restrict__ bnd, const float4* __restrict__ cols,
     const float* __restrict__ Bz, const uint8_t* __restrict__ mask, Dims d, BpConsts c,
     int vb, int vs, int vc, const float* __restrict__ invCol, uint16_t* __restrict__ shadow) {
   const int lane = threadIdx.x & 63;
@@ -934,57 +1044,59 @@ __global__ __launch_bounds__(kBlock) void bp_matched_kernel(
     const ViewP vp = views[p];
     const int ii = vp.xmarch ? x : y;
     const int j = vp.xmarch ? y : x;
-    const float fi = (float)ii;
-    const float2* bt = bnd + (size_t)p * (DNU + 1);
+    const double fi = (double)ii;
+    const double2* bt = bnd + (size_t)p * (DNU + 1);
     const float4* ctab = cols + (size_t)p * DNU;
-    const float dir = (float)vp.bdir;
-    const float cl = (vp.bdir > 0) ? (float)j - 0.5f : -((float)j + 0.5f);
-    const float ch = cl + 1.0f;
+    const double dir = (double)vp.bdir;
+    const double cl = (vp.bdir > 0) ? (double)j - 0.5 : -((double)j + 0.5);
+    const double ch = cl + 1.0;
     // first guess from the fan angle of the voxel centre, then walk to the first
-    // column whose footprint (same FP32 expressions as fp_kernel) reaches cl
+    // column whose footprint (the FP64 boundary lines of fp_kernel) reaches cl
     const float dX = Px - vp.sx, dY = Py - vp.sy;
     const float beta = asinf((vp.sy * dX - vp.sx * dY) * c.inv_sid * rsqrtf(dX * dX + dY * dY));
     int u = (int)fminf(fmaxf(floorf((beta - d.beta0) / d.dbeta_g), 0.f), (float)(DNU - 1));
-    float su = dir * fmaf(fi, bt[u].y, bt[u].x);
+    double su = dir * fma(fi, bt[u].y, bt[u].x);
     while (u > 0 && su > cl) {
       --u;
-      su = dir * fmaf(fi, bt[u].y, bt[u].x);
+      su = dir * fma(fi, bt[u].y, bt[u].x);
     }
-    float su1 = dir * fmaf(fi, bt[u + 1].y, bt[u + 1].x);
+    double su1 = dir * fma(fi, bt[u + 1].y, bt[u + 1].x);
     while (u < DNU - 1 && su1 <= cl) {
       ++u;
       su = su1;
-      su1 = dir * fmaf(fi, bt[u + 1].y, bt[u + 1].x);
+      su1 = dir * fma(fi, bt[u + 1].y, bt[u + 1].x);
     }
     const float* yk = yp + (size_t)k * frame;
+    // z relative to azi, as fp_kernel
+    const float az = vp.azf;
     for (; u < DNU && su < ch;) {
-      const float wy = fminf(su1, ch) - fmaxf(su, cl);
-      if (wy > 0.f) {
+      const double wy = fmin(su1, ch) - fmax(su, cl);
+      if (wy > 0.0) {
         const float4 ct = ctab[u];
-        const float r = fmaf(fi, ct.y, ct.x);
-        const float wyi = wy * __builtin_amdgcn_rcpf((su1 - su) * r);
+        const float r = fmaf((float)ii, ct.y, ct.x);
+        const float wyi = (float)wy * __builtin_amdgcn_rcpf((float)(su1 - su) * r);
         const float* yu = yk + (size_t)u * DNV;
         const float rinv = 1.0f / r;
-        const float zlo0 = fmaf(r, Bz[0], vp.az), zhiN = fmaf(r, Bz[DNV], vp.az);
+        const float zlo0 = fmaf(r, Bz[0], az), zhiN = fmaf(r, Bz[DNV], az);
 #pragma unroll
         for (int g = 0; g < ZG; ++g) {
           // rows whose z footprint [az + Bz[v] r, az + Bz[v+1] r] meets [zlo, zhi]
-          const float zlo = (float)(zb + 64 * g) - 0.5f, zhi = zlo + 1.0f;
+          const float zlo = (float)(zb + 64 * g - vp.azi) - 0.5f, zhi = zlo + 1.0f;
           const bool vis = live[g] && zhi > zlo0 && zlo < zhiN;
           if (!__any(vis)) continue;
           if (!vis) continue;
-          int v = (int)fminf(fmaxf(floorf(((zlo - vp.az) * rinv - d.B0) / d.dB), 0.f),
+          int v = (int)fminf(fmaxf(floorf(((zlo - az) * rinv - d.B0) / d.dB), 0.f),
                              (float)(DNV - 1));
-          float zl = fmaf(r, Bz[v], vp.az);
+          float zl = fmaf(r, Bz[v], az);
           while (v > 0 && zl > zlo) {
             --v;
-            zl = fmaf(r, Bz[v], vp.az);
+            zl = fmaf(r, Bz[v], az);
           }
-          float zh = fmaf(r, Bz[v + 1], vp.az);
+          float zh = fmaf(r, Bz[v + 1], az);
           while (v < DNV - 1 && zh <= zlo) {
             ++v;
             zl = zh;
-            zh = fmaf(r, Bz[v + 1], vp.az);
+            zh = fmaf(r, Bz[v + 1], az);
           }
           float gsum = 0.f;
           for (; v < DNV && zl < zhi;) {
@@ -993,7 +1105,7 @@ __global__ __launch_bounds__(kBlock) void bp_matched_kernel(
             ++v;
             if (v < DNV) {
               zl = zh;
-              zh = fmaf(r, Bz[v + 1], vp.az);
+              zh = fmaf(r, Bz[v + 1], az);
             }
           }
           acc[g] = fmaf(wyi, gsum, acc[g]);
@@ -1002,7 +1114,7 @@ __global__ __launch_bounds__(kBlock) void bp_matched_kernel(
       ++u;
       if (u < DNU) {
         su = su1;
-        su1 = dir * fmaf(fi, bt[u + 1].y, bt[u + 1].x);
+        su1 = dir * fma(fi, bt[u + 1].y, bt[u + 1].x);
       }
     }
   }
@@ -1128,7 +1240,7 @@ __device__ __forceinline__ float matched_cols(const float* __restrict__ yk,
 template <bool UPDATE>
 __global__ __launch_bounds__(kBlock) void bp_matched_win_kernel(
     const float* __restrict__ yp, float* __restrict__ vol, const ViewP* __restrict__ views,
-    const float2* __restrict__ bnd, const float4* __restrict__ cols,
+    const double2* __restrict__ bnd, const float4* __restrict__ cols,
     const float* __restrict__ Bz, const uint8_t* __restrict__ mask, Dims d, BpConsts c,
     int vb, int vs, int vc, const float* __restrict__ invCol, uint16_t* __restrict__ shadow) {
   extern __shared__ float lds_m[];
@@ -1162,34 +1274,35 @@ __global__ __launch_bounds__(kBlock) void bp_matched_win_kernel(
     float q_w[4] = {0.f, 0.f, 0.f, 0.f}, q_r[4] = {1.f, 1.f, 1.f, 1.f};
     float q_ig[4] = {0.f, 0.f, 0.f, 0.f};
     float q_az = 0.f;
+    int q_azi = 0;
     int q_nr = 0;
     if (kl < vc) {
       const int p = vb + kl * vs;
       const ViewP vp = views[p];
       const int ii = vp.xmarch ? x : y;
       const int j = vp.xmarch ? y : x;
-      const float fi = (float)ii;
-      const float2* bt = bnd + (size_t)p * (DNU + 1);
+      const double fi = (double)ii;
+      const double2* bt = bnd + (size_t)p * (DNU + 1);
       const float4* ctab = cols + (size_t)p * DNU;
-      const float dir = (float)vp.bdir;
-      const float cl = (vp.bdir > 0) ? (float)j - 0.5f : -((float)j + 0.5f);
-      const float ch = cl + 1.0f;
+      const double dir = (double)vp.bdir;
+      const double cl = (vp.bdir > 0) ? (double)j - 0.5 : -((double)j + 0.5);
+      const double ch = cl + 1.0;
       // first guess from the fan angle of the voxel centre, then walk to the first
-      // column whose footprint (same FP32 expressions as fp_kernel) reaches cl
+      // column whose footprint (the FP64 boundary lines of fp_kernel) reaches cl
       const float dX = Px - vp.sx, dY = Py - vp.sy;
       const float beta =
           asinf((vp.sy * dX - vp.sx * dY) * c.inv_sid * rsqrtf(dX * dX + dY * dY));
       int u = (int)fminf(fmaxf(floorf((beta - d.beta0) / d.dbeta_g), 0.f), (float)(DNU - 1));
-      float su = dir * fmaf(fi, bt[u].y, bt[u].x);
+      double su = dir * fma(fi, bt[u].y, bt[u].x);
       while (u > 0 && su > cl) {
         --u;
-        su = dir * fmaf(fi, bt[u].y, bt[u].x);
+        su = dir * fma(fi, bt[u].y, bt[u].x);
       }
-      float su1 = dir * fmaf(fi, bt[u + 1].y, bt[u + 1].x);
+      double su1 = dir * fma(fi, bt[u + 1].y, bt[u + 1].x);
       while (u < DNU - 1 && su1 <= cl) {
         ++u;
         su = su1;
-        su1 = dir * fmaf(fi, bt[u + 1].y, bt[u + 1].x);
+        su1 = dir * fma(fi, bt[u + 1].y, bt[u + 1].x);
       }
       // the columns u .. while su < ch (zero-overlap ones included: they add nothing)
       q_u0 = u;
@@ -1197,10 +1310,10 @@ __global__ __launch_bounds__(kBlock) void bp_matched_win_kernel(
       int n = 0;
       for (; u < DNU && su < ch;) {
         const float4 ct = ctab[u];
-        const float r = fmaf(fi, ct.y, ct.x);
-        const float wy = fminf(su1, ch) - fmaxf(su, cl);
+        const float r = fmaf((float)ii, ct.y, ct.x);
+        const double wy = fmin(su1, ch) - fmax(su, cl);
         if (n < 4) {
-          q_w[n] = wy > 0.f ? wy * __builtin_amdgcn_rcpf((su1 - su) * r) : 0.f;
+          q_w[n] = wy > 0.0 ? (float)wy * __builtin_amdgcn_rcpf((float)(su1 - su) * r) : 0.f;
           q_r[n] = r;
           q_ig[n] = __builtin_amdgcn_rcpf(r * d.dB);
         }
@@ -1210,20 +1323,23 @@ __global__ __launch_bounds__(kBlock) void bp_matched_win_kernel(
         ++u;
         if (u < DNU) {
           su = su1;
-          su1 = dir * fmaf(fi, bt[u + 1].y, bt[u + 1].x);
+          su1 = dir * fma(fi, bt[u + 1].y, bt[u + 1].x);
         }
       }
       q_nu = n;
-      q_az = vp.az;
+      // z relative to azi, as fp_kernel
+      q_az = vp.azf;
+      q_azi = vp.azi;
       if (n > 0) {
         // z cells [k - .5, k + .5] meeting [az + r Bz[0], az + r Bz[DNV]] for some r of
         // the columns (r > 0 beyond the source)
-        const float lo = fminf(fmaf(rmin, Bz[0], vp.az), fmaf(rmax, Bz[0], vp.az));
-        const float hi = fmaxf(fmaf(rmin, Bz[DNV], vp.az), fmaf(rmax, Bz[DNV], vp.az));
-        const float za = fminf(fmaxf(lo - 0.5f, -4.f), ZNf + 4.f);
-        const float zb = fminf(fmaxf(hi + 0.5f, -4.f), ZNf + 4.f);
-        q_z0 = max((int)floorf(za) - 1, 0);
-        q_z1 = min((int)ceilf(zb) + 1, d.ZN - 1);
+        const float lo = fminf(fmaf(rmin, Bz[0], vp.azf), fmaf(rmax, Bz[0], vp.azf));
+        const float hi = fmaxf(fmaf(rmin, Bz[DNV], vp.azf), fmaf(rmax, Bz[DNV], vp.azf));
+        const float fazi = (float)vp.azi;
+        const float za = fminf(fmaxf(lo - 0.5f, -4.f - fazi), (ZNf + 4.f) - fazi);
+        const float zb = fminf(fmaxf(hi + 0.5f, -4.f - fazi), (ZNf + 4.f) - fazi);
+        q_z0 = max((int)floorf(za) - 1 + vp.azi, 0);
+        q_z1 = min((int)ceilf(zb) + 1 + vp.azi, d.ZN - 1);
         // rows of spacing >= h meeting a cell of height 1: at most ceil(1 / h) + 1; h is
         // the smallest spacing rmin dBmin less the rounding of boundaries of size <= big
         const float big = fmaxf(fmaxf(fabsf(lo), fabsf(hi)), 1.f);
@@ -1239,6 +1355,7 @@ __global__ __launch_bounds__(kBlock) void bp_matched_win_kernel(
       const int u0 = __builtin_amdgcn_readlane(q_u0, jl), nu = __builtin_amdgcn_readlane(q_nu, jl);
       const int z0 = __builtin_amdgcn_readlane(q_z0, jl), z1 = __builtin_amdgcn_readlane(q_z1, jl);
       const float az = rdl(q_az, jl);
+      const int azi = __builtin_amdgcn_readlane(q_azi, jl);
       const int nr = __builtin_amdgcn_readlane(q_nr, jl);
       const float* yk = yp + (size_t)k * frame + (size_t)u0 * DNV;
       if (nu <= 4 && nr <= 5) {
@@ -1253,7 +1370,7 @@ __global__ __launch_bounds__(kBlock) void bp_matched_win_kernel(
         for (int zz = z0; zz <= z1; zz += 64) {
           const int z = zz + lane;
           if (z > z1) continue;
-          const float zlo = (float)z - 0.5f;
+          const float zlo = (float)(z - azi) - 0.5f;
           const float a0 = lacc[z];
           float a;
           switch (nr) {
@@ -1274,7 +1391,7 @@ __global__ __launch_bounds__(kBlock) void bp_matched_win_kernel(
         for (int zz = z0; zz <= z1; zz += 64) {
           const int z = zz + lane;
           if (z > z1) continue;
-          const float zlo = (float)z - 0.5f;
+          const float zlo = (float)(z - azi) - 0.5f;
           float a = lacc[z];
 #pragma unroll
           for (int t = 0; t < 4; ++t) {
@@ -1290,26 +1407,26 @@ __global__ __launch_bounds__(kBlock) void bp_matched_win_kernel(
         const ViewP vp = views[p];
         const int ii = vp.xmarch ? x : y;
         const int j = vp.xmarch ? y : x;
-        const float fi = (float)ii;
-        const float2* bt = bnd + (size_t)p * (DNU + 1);
+        const double fi = (double)ii;
+        const double2* bt = bnd + (size_t)p * (DNU + 1);
         const float4* ctab = cols + (size_t)p * DNU;
-        const float dir = (float)vp.bdir;
-        const float cl = (vp.bdir > 0) ? (float)j - 0.5f : -((float)j + 0.5f);
-        const float ch = cl + 1.0f;
+        const double dir = (double)vp.bdir;
+        const double cl = (vp.bdir > 0) ? (double)j - 0.5 : -((double)j + 0.5);
+        const double ch = cl + 1.0;
         for (int zz = z0; zz <= z1; zz += 64) {
           const int z = zz + lane;
           if (z > z1) continue;
-          const float zlo = (float)z - 0.5f;
+          const float zlo = (float)(z - azi) - 0.5f;
           float a = lacc[z];
           for (int t = 0; t < nu; ++t) {
             const int u = u0 + t;
-            const float su = dir * fmaf(fi, bt[u].y, bt[u].x);
-            const float su1 = dir * fmaf(fi, bt[u + 1].y, bt[u + 1].x);
-            const float wy = fminf(su1, ch) - fmaxf(su, cl);
-            if (wy > 0.f) {
+            const double su = dir * fma(fi, bt[u].y, bt[u].x);
+            const double su1 = dir * fma(fi, bt[u + 1].y, bt[u + 1].x);
+            const double wy = fmin(su1, ch) - fmax(su, cl);
+            if (wy > 0.0) {
               const float4 ct = ctab[u];
-              const float r = fmaf(fi, ct.y, ct.x);
-              const float wyi = wy * __builtin_amdgcn_rcpf((su1 - su) * r);
+              const float r = fmaf((float)ii, ct.y, ct.x);
+              const float wyi = (float)wy * __builtin_amdgcn_rcpf((float)(su1 - su) * r);
               a = fmaf(wyi, matched_rows(yk + t * DNV, sBz, DNV, r, az, zlo, d.B0, d.dB), a);
             }
           }
@@ -1475,8 +1592,9 @@ int fp_partial_blocks(const Plan& P, int vc) {
 template <typename TV, typename TO>
 FpArgs<TV, TO> fp_args(const Plan& P, const TV* vol, TO* out, int vb, int vs, const float* b,
                        const float* invRow, double* partials) {
-  return FpArgs<TV, TO>{vol,    out, P.views, P.bnd, P.cols, P.Bz,   P.h2,    P.invdB,
-                        P.maskf, P.mext, P.d,  vb,    vs,     b,      invRow, partials};
+  return FpArgs<TV, TO>{vol,     out,    P.views, P.bnd, P.bnd64, P.cols,  P.Bz,
+                        P.h2,    P.invdB, P.maskf, P.mext, P.d,   vb,      vs,
+                        b,       invRow, partials};
 }
 
 template <typename T>
@@ -1543,17 +1661,17 @@ int launch_bp(const Plan& P, const T* prj, float* vol, int vb, int vs, int vc, i
         const dim3 wgrid((P.d.XN * P.d.YN + kWaves - 1) / kWaves);
         if (accumulate)
           hipLaunchKernelGGL((bp_matched_win_kernel<true>), wgrid, dim3(kBlock), lds, st, prj,
-                             vol, P.views, P.bnd, P.cols, P.Bz, P.mask, P.d, P.bc, vb, vs, vc,
+                             vol, P.views, P.bnd64, P.cols, P.Bz, P.mask, P.d, P.bc, vb, vs, vc,
                              invCol, shadow);
         else
           hipLaunchKernelGGL((bp_matched_win_kernel<false>), wgrid, dim3(kBlock), lds, st, prj,
-                             vol, P.views, P.bnd, P.cols, P.Bz, P.mask, P.d, P.bc, vb, vs, vc,
+                             vol, P.views, P.bnd64, P.cols, P.Bz, P.mask, P.d, P.bc, vb, vs, vc,
                              nullptr, nullptr);
         return check_launch("bp_matched_win_kernel");
       }
 #define CTR_BPM(UPD, ZGV, IC, SH)                                                                \
   hipLaunchKernelGGL((bp_matched_kernel<UPD, ZGV>), rgrid, dim3(kBlock), 0, st, prj, vol, P.views, \
-                     P.bnd, P.cols, P.Bz, P.mask, P.d, P.bc, vb, vs, vc, IC, SH)
+                     P.bnd64, P.cols, P.Bz, P.mask, P.d, P.bc, vb, vs, vc, IC, SH)
       if (accumulate) {
         switch (zg) {
           case 1: CTR_BPM(true, 1, invCol, shadow); break;
@@ -1577,7 +1695,7 @@ int launch_bp(const Plan& P, const T* prj, float* vol, int vb, int vs, int vc, i
                                  accumulate ? shadow : nullptr);
   if (P.d.ZN <= kWinMaxZ) {
     const dim3 wgrid((P.d.XN * P.d.YN + kWaves - 1) / kWaves);
-    const size_t stg = sizeof(T) == 2 ? kStageViews * kStagePad : 0;  // staged rows: BF16
+    const size_t stg = P.d.DNV <= kStageRows ? kStageF : 0;  // staged views
     const size_t lds = (size_t)kWaves * (P.d.ZN + stg) * sizeof(float);
     if (accumulate)
       hipLaunchKernelGGL((bp_win_kernel<true, T>), wgrid, dim3(kBlock), lds, st, A);
@@ -1649,20 +1767,20 @@ int launch_bp_batch(const BpBatchItem<T>* items, int nscan, int total, int zg, i
 }
 
 template <typename T>
-int launch_bp_batch_win(const BpBatchItem<T>* items, int nscan, int total, int zmax, int grid,
-                        int strided, hipStream_t st) {
+int launch_bp_batch_win(const BpBatchItem<T>* items, int nscan, int total, int zmax,
+                        int maxDNV, int grid, int strided, hipStream_t st) {
   if (total <= 0) return CTR_OK;
   if (zmax < 1 || zmax > kWinMaxZ) return fail(CTR_ERR_ARG, "bp_batch_win: column height");
   const int sched = grid <= 0 ? 2 : (strided ? 1 : 0);
   grid = grid <= 0 ? total : std::max(1, std::min(grid, total));
-  const size_t stg = sizeof(T) == 2 ? kStageViews * kStagePad : 0;  // staged rows: BF16
-  const size_t lds = (size_t)kWaves * (zmax + stg) * sizeof(float);
+  const int staged = maxDNV <= kStageRows ? 1 : 0;
+  const size_t lds = (size_t)kWaves * (zmax + (staged ? kStageF : 0)) * sizeof(float);
   if (sched == 2)
     hipLaunchKernelGGL((bp_batch_win1_kernel<T>), dim3(grid), dim3(kBlock), lds, st, items,
-                       nscan, total, zmax);
+                       nscan, total, zmax, staged);
   else
     hipLaunchKernelGGL((bp_batch_win_kernel<T>), dim3(grid), dim3(kBlock), lds, st, items,
-                       nscan, total, sched, zmax);
+                       nscan, total, sched, zmax, staged);
   return check_launch("bp_batch_win_kernel");
 }
 
@@ -1708,9 +1826,9 @@ template int launch_fp_batch<uint16_t>(const FpBatchItem<uint16_t>*, int, int, b
 template int launch_bp_batch<float>(const BpBatchItem<float>*, int, int, int, int, int,
                                     hipStream_t);
 template int launch_bp_batch_win<float>(const BpBatchItem<float>*, int, int, int, int, int,
-                                        hipStream_t);
+                                        int, hipStream_t);
 template int launch_bp_batch_win<uint16_t>(const BpBatchItem<uint16_t>*, int, int, int, int,
-                                           int, hipStream_t);
+                                           int, int, hipStream_t);
 template int launch_bp_batch<uint16_t>(const BpBatchItem<uint16_t>*, int, int, int, int, int,
                                        hipStream_t);
 template int launch_scale_matched<float>(const Plan&, const float*, float*, int, int, int,

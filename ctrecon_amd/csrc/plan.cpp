@@ -208,6 +208,7 @@ static int build_tables(const ctr_geometry* g, Plan& P, double half = 0.0) {
   for (int u = 0; u < DNU; ++u) det64[u] = make_double2(g->xds[u], g->yds[u]);
   for (int v = 0; v < DNV; ++v) zc[v] = (float)((g->zds[v] - z0) / dz);
   std::vector<float2> bnd((size_t)PN * (DNU + 1));
+  std::vector<double2> bnd64((size_t)PN * (DNU + 1));
   std::vector<float4> cols((size_t)PN * DNU);
   for (int p = 0; p < PN; ++p) {
     const double th = regularize((double)g->angles[p]);
@@ -218,10 +219,13 @@ static int build_tables(const ctr_geometry* g, Plan& P, double half = 0.0) {
     src64[p] = make_double2(sx, sy);
     rot64[p] = make_double2(c, s);
     sz64[p] = sz;
-    vp.az = (float)(sz / dz + ocz + half);
+    const double az = sz / dz + ocz + half;
+    vp.az = (float)az;
+    vp.azi = (int)std::floor(az + 0.5);
+    vp.azf = (float)(az - vp.azi);
+    vp.pad0 = vp.pad1 = vp.pad2 = 0;
     vp.xmarch = std::fabs(c) <= std::fabs(s) ? 1 : 0;  // DD3_GPU.cu:914
     vp.wx = std::fabs(sx) > std::fabs(sy) ? 1 : 0;  // DD3_GPU.cu:1453
-    vp.pad1 = 0;
     const int NS = vp.xmarch ? g->XN : g->YN;
     const double mid = 0.5 * (NS - 1);
     for (int b = 0; b <= DNU; ++b) {
@@ -235,6 +239,7 @@ static int build_tables(const ctr_geometry* g, Plan& P, double half = 0.0) {
         a = ((-ocy * dx - By) * f + Bx) / dx + ocx + half;
       }
       bnd[(size_t)p * (DNU + 1) + b] = make_float2((float)a, (float)f);
+      bnd64[(size_t)p * (DNU + 1) + b] = make_double2(a, f);
     }
     {
       const float2 b0 = bnd[(size_t)p * (DNU + 1)], bN = bnd[(size_t)p * (DNU + 1) + DNU];
@@ -302,6 +307,7 @@ static int build_tables(const ctr_geometry* g, Plan& P, double half = 0.0) {
   if ((rc = upload(&P.det64, det64))) return rc;
   if ((rc = upload(&P.zc, zc))) return rc;
   if ((rc = upload(&P.bnd, bnd))) return rc;
+  if ((rc = upload(&P.bnd64, bnd64))) return rc;
   if ((rc = upload(&P.cols, cols))) return rc;
   if ((rc = upload(&P.Bz, Bz))) return rc;
   if ((rc = upload(&P.h2, h2))) return rc;
@@ -320,6 +326,7 @@ static void free_plan(Plan* P) {
   (void)hipFree(P->det64);
   (void)hipFree(P->zc);
   (void)hipFree(P->bnd);
+  (void)hipFree(P->bnd64);
   (void)hipFree(P->cols);
   (void)hipFree(P->Bz);
   (void)hipFree(P->h2);

@@ -2,8 +2,9 @@
 100-115: volumes U{256..512}^2 x U{64..192}, voxels U(0.5, 1.0) mm, pitch U(0.5, 1.5),
 2-4 turns; 888 x 64 detector, 984 views/turn), against the FP64 oracle.
 
-* Every scan's forward projection on sampled views of its first OS-SART subset (views 0
-  and 20 of 20 subsets) within 1e-5 of the oracle.
+* Every scan's forward projection on sampled views of its first OS-SART subset (view 0
+  and one half-way through the scan, of 20 subsets) within 1e-5 of the oracle; a view
+  whose cone misses the volume projects to exact zeros.
 * The full batch with BF16 storage (tuned launch shapes, graph replay): two sweeps lower
   every scan's residual, and agree with each scan's own OS-SART session.
 * One batched subset update per scan against the oracle's FP64 OS-SART step, on four
@@ -53,9 +54,15 @@ def test_config4_forward_sampled_views(scans, oracle, gpu_plan_factory):
     for seed in SEEDS:
         g, vol, pitch = scans[seed]
         plan = gpu_plan_factory(g)
-        for p in (0, 20):
+        # the first view and one half-way through the scan, both of the first subset
+        for p in (0, 20 * (g.PN // 40)):
             got = plan.forward(vol, [p])
             ref = oracle.fp(g.subset([p]), vol, "direct")
+            if not np.any(ref):
+                # a helical scan may start or end with its cone outside the volume
+                _say(f"  scan {seed} view {p}: the cone misses the volume")
+                assert not np.any(got)
+                continue
             err = rel_l2(got, ref)
             _say(f"  scan {seed} ({g.XN}x{g.YN}x{g.ZN}, pitch {pitch:.2f}) view {p}: "
                  f"fp rel-L2 {err:.3e}")
