@@ -37,7 +37,7 @@ EXPORTS = [
     "ctr_plan_set_storage", "ctr_plan_mask_volume", "ctr_plan_fp", "ctr_plan_fp_bf16", "ctr_plan_bp_bf16",
     "ctr_f32_to_bf16", "ctr_bf16_to_f32",
     "ctr_plan_bp", "ctr_os_sart", "ctr_os_sart_run", "ctr_cone_voxels", "ctr_conjgrad",
-    "ctr_residual", "ctr_device_count",
+    "ctr_residual", "ctr_residual_noalloc", "ctr_device_count",
     "ctr_malloc", "ctr_free", "ctr_memcpy_h2d", "ctr_memcpy_d2h", "ctr_memset",
     "ctr_synchronize", "ctr_device_copy", "ctr_device_sub", "ctr_device_axpy", "ctr_last_error", "ctr_version",
     "ctr_sart_create", "ctr_sart_sweeps", "ctr_sart_setup_ms", "ctr_sart_destroy",
@@ -117,6 +117,7 @@ def lib():
         L.ctr_cone_voxels.argtypes = [p, i, i, i, p]
         L.ctr_conjgrad.argtypes = [p, p, p, i, i, p]
         L.ctr_residual.argtypes = [p, p, p, ctypes.POINTER(d)]
+        L.ctr_residual_noalloc.argtypes = [p, p, p, ctypes.POINTER(d)]
         L.ctr_device_count.argtypes = [ctypes.POINTER(i)]
         L.ctr_malloc.argtypes = [ctypes.POINTER(p), ctypes.c_ulonglong, i]
         L.ctr_free.argtypes = [p]

@@ -232,6 +232,17 @@ int ctr_conjgrad(ctr_plan* plan, float* d_x, const float* d_prj, int iterNum, in
 /* Relative data residual ||A x - b|| / ||b|| over all views (FP64 sums). Synchronous. */
 int ctr_residual(ctr_plan* plan, const float* d_x, const float* d_prj, double* out);
 
+/* ctr_residual for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc); no counterpart in the
+ * reference.  Same result as ctr_residual, bit for bit, and it calls no hipMalloc / hipFree:
+ * its reduction scratch is a fixed 4 MiB device array of the library (one per GPU, held
+ * until the process ends), enough for detectors of DNU * DNV up to 261888 rays when the
+ * scan has 256 views or more (more rays with fewer views; CTR_ERR_ARG beyond).  Direct
+ * method only (CTR_ERR_UNSUPPORTED otherwise).  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (several threads or plans at
+ * once) race and must be serialised by the caller.  Synchronous. */
+int ctr_residual_noalloc(ctr_plan* plan, const float* d_x, const float* d_prj, double* out);
+
 /* ---- batched OS-SART over several scans (build addition, no reference counterpart) ----
  * nscan scans of any sizes (one plan each, same device and storage, direct method),
  * their iterates d_x[i] and measured sinograms d_prj[i] (kept by the batch until
