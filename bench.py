@@ -100,7 +100,7 @@ def cpu_baseline(g, vol, n_views, target_sweeps, setup_sweeps=1.0):
 
 
 def headline_record(args, plain, learned, learned_sino, roof, pps, kernels, cpu, have_net,
-                    views):
+                    views, recipe=None):
     """The one-line JSON record of the clinical workload.  `plain` / `learned` are the legs'
     result dicts (learned None without trained weights); `plain["timed_ms"]` holds the K
     timed sweeps."""
@@ -141,6 +141,8 @@ def headline_record(args, plain, learned, learned_sino, roof, pps, kernels, cpu,
         "plain_fp32": {k: v for k, v in plain.items() if k != "timed_ms"},
         "learned_bf16": learned,
         "learned_bf16_sinogram_only": learned_sino,
+        # the configured recipe's network beside the shipped ones (not the reported value)
+        "learned_bf16_configured_recipe": recipe,
         "speedup_vs_plain": t_plain / t_learned if both else None,
         "rmse_ratio_vs_plain": (learned["rmse_vs_truth_in_mask"] / plain["rmse_vs_truth_in_mask"]
                                 if learned is not None else None),
@@ -317,6 +319,12 @@ def parse_args(argv=None):
     ap.add_argument("--after2", type=int, default=3)
     ap.add_argument("--weights", default=os.path.join(ROOT, "ctrecon_amd", "weights",
                                                       "warm_start.npz"))
+    ap.add_argument("--weights-recipe", default=os.path.join(ROOT, "ctrecon_amd", "weights",
+                                                             "warm_start_iter50.npz"),
+                    help="network of the configured training recipe")
+    ap.add_argument("--recipe", action="store_true",
+                    help="also run the configured recipe's network to the target (capped at "
+                         "the plain sweep count) and report it beside the shipped networks")
     ap.add_argument("--workload", default="clinical",
                     choices=["clinical", "batched", "projectors"],
                     help="clinical: the headline solve; batched: 16 uneven scans, one sweep; "
@@ -438,7 +446,7 @@ def main():
         f"{np.mean(plain['timed_ms']):.1f} ms each")
 
     # ---- the learned warm start with BF16 operands, from zero to the target
-    learned = learned_sino = None
+    learned = learned_sino = recipe = None
     if run_learned:
         from ctrecon_amd.learned import warm_start_os_sart
         from ctrecon_amd.unet import UNet
@@ -448,7 +456,7 @@ def main():
         if os.path.exists(args.weights2):
             stages.append((UNet.from_file(args.weights2), args.after2))
 
-        def learned_leg(storage):
+        def learned_leg(storage, net=net, stages=stages, weights=args.weights, cap=None):
             plan.set_storage(storage)
             # warm-up: one sweep and one network pass (code loading, network buffers)
             d_x.zero()
@@ -461,7 +469,7 @@ def main():
             plan.synchronize()
             t0 = time.perf_counter()
             info = warm_start_os_sart(plan, net, d_b, d_x, args.subsets, args.target,
-                                      max_sweeps, args.bp, stages=stages)
+                                      cap or max_sweeps, args.bp, stages=stages)
             plan.synchronize()
             wall = time.perf_counter() - t0
             plan.set_storage("f32")
@@ -478,9 +486,11 @@ def main():
                 "residual_sweep_final": float(r[-1]),
                 "residual_true_final": plan.residual(d_x, d_b),
                 "rmse_vs_truth_in_mask": rmse_of(d_x),
-                "weights": os.path.relpath(args.weights, ROOT),
+                "weights": os.path.relpath(weights, ROOT),
                 "later_stages": [os.path.relpath(args.weights2, ROOT)] if stages else [],
             }
+            if cap:
+                out["max_sweeps"] = cap
             log(f"learned + {storage}: {info['sweeps']} sweeps, wall {wall:.2f} s, "
                 f"residual {r[-1]:.3e}, fallback {info['fallback']}")
             return out
@@ -489,6 +499,13 @@ def main():
         # operands with the FP32 iterate
         learned = learned_leg("bf16")
         learned_sino = learned_leg("bf16_sino")
+        # the network of the configured training recipe (targets: 50-sweep OS-SART iterates,
+        # 256x256x128 training grid), first stage alone, capped at the plain sweep count
+        # (time_to_target_s null: not faster than plain)
+        if args.recipe and os.path.exists(args.weights_recipe):
+            recipe = learned_leg("bf16", net=UNet.from_file(args.weights_recipe), stages=[],
+                                 weights=args.weights_recipe,
+                                 cap=plain["sweeps_to_target"] or max_sweeps)
 
     # ---- roofline of the dominant kernel (the back-projection update)
     if st is None:  # no warm-up sweeps: time two sweeps' launches now
@@ -540,7 +557,7 @@ def main():
         log("cpu baseline sample")
         cpu = cpu_baseline(g, x_true, args.cpu_views, plain["sweeps_to_target"])
     out = headline_record(args, plain, learned, learned_sino, roof, pps, kernels, cpu,
-                          have_net, g.PN)
+                          have_net, g.PN, recipe)
     print(json.dumps(out))
 
 

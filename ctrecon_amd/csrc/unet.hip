@@ -133,6 +133,95 @@ __global__ __launch_bounds__(256) void conv3_kernel(const uint16_t* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// The same convolution with its input staged in LDS.  A workgroup computes a box of
+// kTY x kTX x kTZ = 4 x 4 x 16 output voxels (wave w: line y0 + w, whose four x lines are
+// its four 16-row MFMA blocks); the box's input halo, 6 x 6 x 18 voxels with zeros
+// outside the grid, is staged CC channels at a time, so every input element is read
+// from memory about 2.5 times per output instead of 27.  K runs over the channel
+// chunks, and within a chunk over k = tap * CC + c as in conv3_kernel.
+// ---------------------------------------------------------------------------
+constexpr int kTY = 4, kTX = 4, kTZ = 16;
+constexpr int kHX = kTX + 2, kHZ = kTZ + 2;
+constexpr int kHV = (kTY + 2) * kHX * kHZ;  // voxels of the halo box
+
+template <int NB, int CC>
+__global__ __launch_bounds__(256) void conv3_lds_kernel(
+    const uint16_t* __restrict__ in, int cs_in, int Cin, const uint16_t* __restrict__ W,
+    const float* __restrict__ bias, uint16_t* __restrict__ out, int cs_out, int relu, Grid3 g,
+    int tiles_x, int tiles_z) {
+  constexpr int Cout = 16 * NB;
+  constexpr int VS = CC + 8;  // channels per staged voxel, padded against bank conflicts
+  constexpr int U = CC / 8;   // 16-byte units per staged voxel
+  __shared__ __align__(16) uint16_t halo[kHV * VS];
+  const int lane = threadIdx.x & 63, row = lane & 15, grp = lane >> 4;
+  const int wv = threadIdx.x >> 6;
+  int t = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int tz = t % tiles_z;
+  t /= tiles_z;
+  const int tx = t % tiles_x, ty = t / tiles_x;
+  const int y0 = ty * kTY, x0 = tx * kTX, z0 = tz * kTZ;
+  f32x4 acc[4][NB];
+#pragma unroll
+  for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int c0 = 0; c0 < Cin; c0 += CC) {
+    for (int i = threadIdx.x; i < kHV * U; i += 256) {
+      const int hv = i / U, u = i - hv * U;
+      const int hz = hv % kHZ, hxy = hv / kHZ;
+      const int hx = hxy % kHX, hy = hxy / kHX;
+      const int yy = y0 - 1 + hy, xx = x0 - 1 + hx, zz = z0 - 1 + hz;
+      uint4 val = make_uint4(0, 0, 0, 0);
+      if (yy >= 0 && yy < g.Y && xx >= 0 && xx < g.X && zz >= 0 && zz < g.Z)
+        val = *(const uint4*)(in + (((long long)yy * g.X + xx) * g.Z + zz) * cs_in + c0 + 8 * u);
+      *(uint4*)(halo + hv * VS + 8 * u) = val;
+    }
+    __syncthreads();
+#pragma unroll 2
+    for (int k0 = 0; k0 < 27 * CC; k0 += 32) {
+      const int kg = k0 + 8 * grp;
+      const int tap0 = kg / CC, c = kg - tap0 * CC;
+      const bool kv = tap0 < 27;  // the last step's padding taps add zeros
+      const int tap = kv ? tap0 : 26;  // ... at addresses inside the arrays
+      const int dy = tap / 9, dx = (tap / 3) % 3, dz = tap % 3;  // offsets into the halo box
+      bf16x8 b[NB];
+#pragma unroll
+      for (int nb = 0; nb < NB; ++nb)
+        b[nb] = kv ? load8(W + ((size_t)tap * Cout + 16 * nb + row) * Cin + c0 + c) : zero8();
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) {
+        const int hv = ((wv + dy) * kHX + (mb + dx)) * kHZ + (row + dz);
+        const bf16x8 a =
+            kv ? __builtin_bit_cast(bf16x8, *(const uint4*)(halo + hv * VS + c)) : zero8();
+#pragma unroll
+        for (int nb = 0; nb < NB; ++nb) acc[mb][nb] = mfma(a, b[nb], acc[mb][nb]);
+      }
+    }
+    __syncthreads();
+  }
+  const int yy = y0 + wv;
+#pragma unroll
+  for (int mb = 0; mb < 4; ++mb) {
+    const int xx = x0 + mb;
+    if (yy >= g.Y || xx >= g.X) continue;
+#pragma unroll
+    for (int nb = 0; nb < NB; ++nb) {
+      const int co = 16 * nb + row;
+      const float bb = bias ? bias[co] : 0.f;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int zz = z0 + 4 * grp + r;
+        if (zz < g.Z) {
+          float val = acc[mb][nb][r] + bb;
+          if (relu) val = fmaxf(val, 0.f);
+          out[(((long long)yy * g.X + xx) * g.Z + zz) * cs_out + co] = to_bf16(val);
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Weight gradient: part[s][tap][co][ci] = sum over the voxels of split s of
 // dY[v][co] * X[v + off(tap)][ci]; for tap 0 also bpart[s][co] = sum dY[v][co].
 // MFMA with rows co (16 * NBO), columns ci (16 * NBI, ci >= Cin masked) and K = 32
@@ -305,6 +394,26 @@ __global__ void upsample_kernel(const uint16_t* __restrict__ in, int cs_in, int 
   }
 }
 
+// upsample_kernel with one workgroup per fine (y, x) line and 8 channels (16 bytes) per
+// copy: channel counts, strides and both pointers 16-byte aligned.
+__global__ __launch_bounds__(256) void upsample_line_kernel(const uint16_t* __restrict__ in,
+                                                            int cs_in, int C, Grid3 gf,
+                                                            uint16_t* __restrict__ out,
+                                                            int cs_out) {
+  const int line = blockIdx.x;  // y * X + x of the fine grid
+  const int y = line / gf.X, x = line - y * gf.X;
+  const long long pline = (long long)(y / 2) * (gf.X / 2) + x / 2;
+  const int U = C / 8;
+  const uint4* src = (const uint4*)in;
+  uint4* dst = (uint4*)out;
+  for (int i = threadIdx.x; i < gf.Z * U; i += blockDim.x) {
+    const int z = i / U, u = i - z * U;
+    const long long p = pline * (gf.Z / 2) + z / 2;
+    const long long v = (long long)line * gf.Z + z;
+    dst[(v * cs_out) / 8 + u] = src[(p * cs_in) / 8 + u];
+  }
+}
+
 // Adjoint of upsample_kernel: din[p][c] = sum of dout over the 8 children.
 __global__ void upsample_bwd_kernel(const uint16_t* __restrict__ dout, int cs_out, int C,
                                     Grid3 gf, uint16_t* __restrict__ din, int cs_in) {
@@ -461,6 +570,33 @@ int conv3(const uint16_t* in, int cs_in, int Cin, const uint16_t* W, const float
           uint16_t* out, int cs_out, int Cout, bool relu, Grid3 g, hipStream_t st) {
   if (Cin % 8 || cs_in % 8 || cs_out % 8 || Cout % 16)
     return fail(CTR_ERR_ARG, "conv3: channel counts and strides must be multiples of 8");
+  const int NB = Cout / 16;
+  if ((Cin == 8 || Cin % 16 == 0) && NB <= 4) {
+    // LDS-staged input halo: 16 channels per stage (8 for an 8-channel input)
+    const int tiles_x = (g.X + kTX - 1) / kTX, tiles_z = (g.Z + kTZ - 1) / kTZ;
+    const long long tiles = (long long)((g.Y + kTY - 1) / kTY) * tiles_x * tiles_z;
+    const dim3 tgrid((unsigned)tiles);
+#define CVL(NBV, CCV)                                                                           \
+  hipLaunchKernelGGL((conv3_lds_kernel<NBV, CCV>), tgrid, dim3(256), 0, st, in, cs_in, Cin, W, \
+                     bias, out, cs_out, (int)relu, g, tiles_x, tiles_z)
+    if (Cin == 8) {
+      switch (NB) {
+        case 1: CVL(1, 8); break;
+        case 2: CVL(2, 8); break;
+        case 3: CVL(3, 8); break;
+        default: CVL(4, 8); break;
+      }
+    } else {
+      switch (NB) {
+        case 1: CVL(1, 16); break;
+        case 2: CVL(2, 16); break;
+        case 3: CVL(3, 16); break;
+        default: CVL(4, 16); break;
+      }
+    }
+#undef CVL
+    return check("conv3_lds_kernel");
+  }
   const long long V = (long long)g.Y * g.X * g.Z;
   const long long waves = (V + 31) / 32;
   dim3 grid((unsigned)((waves + 3) / 4));
@@ -549,6 +685,12 @@ int pool_bwd(const uint16_t* dout, int cs_out, int C, Grid3 gf, uint16_t* din, i
 
 int upsample(const uint16_t* in, int cs_in, int C, Grid3 gf, uint16_t* out, int cs_out,
              hipStream_t st) {
+  if (C % 8 == 0 && cs_in % 8 == 0 && cs_out % 8 == 0 && ((uintptr_t)in & 15) == 0 &&
+      ((uintptr_t)out & 15) == 0 && (long long)gf.Y * gf.X <= 0x7fffffffLL) {
+    hipLaunchKernelGGL(upsample_line_kernel, dim3((unsigned)(gf.Y * gf.X)), dim3(256), 0, st,
+                       in, cs_in, C, gf, out, cs_out);
+    return check("upsample_line_kernel");
+  }
   const long long n = (long long)gf.Y * gf.X * gf.Z * C;
   hipLaunchKernelGGL(upsample_kernel, dim3(grid1(n)), dim3(256), 0, st, in, cs_in, C, gf, out,
                      cs_out);
