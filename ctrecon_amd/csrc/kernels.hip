@@ -942,7 +942,7 @@ __global__ __launch_bounds__(kBlock) void bp_win_ones_kernel(BpArgs<float> A) {
 // has DNV <= kStageRows: staged; otherwise none), then kWaves * zmax floats (zmax: the
 // tallest column of the batch).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void bp_batch_win_kernel(const BpBatchItem<T>* __restrict__ D,
+__global__ __launch_bounds__(kBlock, 7) void bp_batch_win_kernel(const BpBatchItem<T>* __restrict__ D,
                                                               int nscan, int total, int sched,
                                                               int zmax, int staged) {
   extern __shared__ float4 lds_win[];
@@ -962,9 +962,10 @@ __global__ __launch_bounds__(kBlock) void bp_batch_win_kernel(const BpBatchItem<
 }
 
 // bp_batch_win_kernel for the one-workgroup-per-item schedule (grid = total): no item
-// loop, so nothing stays live around the column body.
+// loop, so nothing stays live around the column body.  Both batched windowed kernels are
+// held to 72 VGPRs (7 waves per SIMD), as bp_win_kernel.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void bp_batch_win1_kernel(
+__global__ __launch_bounds__(kBlock, 7) void bp_batch_win1_kernel(
     const BpBatchItem<T>* __restrict__ D, int nscan, int total, int zmax, int staged) {
   extern __shared__ float4 lds_win[];
   float* lds = reinterpret_cast<float*>(lds_win);
