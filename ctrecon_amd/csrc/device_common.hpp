@@ -118,6 +118,10 @@ struct Buf<uint16_t> {
     return __uint_as_float((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, i * 2, j * 2, 0)
                            << 16);
   }
+  // the 8 bytes of elements i .. i + 3 (i even, the array 4-byte aligned)
+  __device__ __forceinline__ uint2 load8b(int i) const {
+    return __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r, i * 2, 0, 0));
+  }
   // 16-bit loads (two aligned 32-bit loads with half-word selects measured slower: the
   // projectors are VALU-bound)
   __device__ __forceinline__ void tap3(int o, int t1, int t2, float& a0, float& a1,

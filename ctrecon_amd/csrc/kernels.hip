@@ -70,17 +70,36 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* slot) {
 // of zero weight reads a neighbouring element of the array, finite, or past its end 0, and
 // adds an exact zero).  NT = 2 when no lane needs a third z cell (the term it drops is an
 // exact zero).  The combining order of the general loop in fp_body.
+// BF16 volumes read each run as the 8 bytes from the even element kb = k0 & ~1 on (one
+// aligned 4-element load; ob even), the three taps picked by a funnel shift by 16 (k0 - kb)
+// bits.
 template <int NJ, int NT, typename TV>
 __device__ __forceinline__ float fp_cells_w(const Buf<TV>& V, int ob, int cS, int k0,
                                             float wz0, float wz1, float wz2, float wy0,
                                             float wy1, float wy2) {
   float a[NJ][3];
+  if constexpr (sizeof(TV) == 2) {
+    const int kb = k0 & ~1;
+    const uint32_t sh = (uint32_t)(k0 & 1) << 4;
+    uint2 w[NJ];
 #pragma unroll
-  for (int ij = 0; ij < NJ; ++ij) {
-    const int o = k0 + (ob + ij * cS);
-    a[ij][0] = V[o];
-    a[ij][1] = V[o + 1];
-    if constexpr (NT == 3) a[ij][2] = V[o + 2];
+    for (int ij = 0; ij < NJ; ++ij) w[ij] = V.load8b(kb + (ob + ij * cS));
+#pragma unroll
+    for (int ij = 0; ij < NJ; ++ij) {
+      // elements k0, k0 + 1 as the low / high halves of x, k0 + 2 as the low half of y
+      const uint32_t x = __builtin_amdgcn_alignbit(w[ij].y, w[ij].x, sh);
+      a[ij][0] = __uint_as_float(x << 16);
+      a[ij][1] = __uint_as_float(x & 0xffff0000u);
+      if constexpr (NT == 3) a[ij][2] = __uint_as_float((w[ij].y >> sh) << 16);
+    }
+  } else {
+#pragma unroll
+    for (int ij = 0; ij < NJ; ++ij) {
+      const int o = k0 + (ob + ij * cS);
+      a[ij][0] = V[o];
+      a[ij][1] = V[o + 1];
+      if constexpr (NT == 3) a[ij][2] = V[o + 2];
+    }
   }
   float s = 0.f;
 #pragma unroll
@@ -153,6 +172,8 @@ __device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk
     const float2* me = A.mext + (vp.xmarch ? 0 : d.XN);
     const int NV = d.XN * d.YN * d.ZN;
     const Buf<TV> V(A.vol, (uint32_t)NV);
+    // BF16 runs as aligned 8-byte loads: even column starts on a 4-byte aligned array
+    const bool bf16_runs = d.ZN % 2 == 0 && (reinterpret_cast<uintptr_t>(A.vol) & 3) == 0;
     const CTR_GLOBAL float* maskf = gp(A.maskf);
     float acc = 0.f;
     // one slice: footprint [clo, chi] x [zl, zh] over the cells j0 .. j0 + nj - 1
@@ -274,10 +295,9 @@ __device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk
           if (!__any(dk > 2.f)) {
             // the fixed runs of fp_cells_w stay inside the array (its merged multi-element
             // loads never straddle the end): not on the array's last column
-            if (sizeof(TV) == 4 && nj <= 3 && ob + (nj - 1) * cS + d.ZN + 2 <= NV) {
-              // nj is uniform: straight-line code per cell count, all loads first (FP32
-              // volumes; with BF16 volumes the sub-dword gathers run faster one cell at
-              // a time)
+            if (nj <= 3 && ob + (nj - 1) * cS + d.ZN + 3 <= NV &&
+                (sizeof(TV) == 4 || bf16_runs)) {
+              // nj is uniform: straight-line code per cell count, all loads first
               if (!__any(dk > 1.f)) {  // at most two z cells in every lane's footprint
                 switch (nj) {
                   case 1: s = fp_cells_w<1, 2>(V, ob, cS, k0, wz0, wz1, wz2, ra.z, ra.w, rb.x); break;
