@@ -98,6 +98,12 @@ struct Buf<float> {
   __device__ __forceinline__ float at(int i, int j) const {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, i * 4, j * 4, 0));
   }
+  // at() as stored bits, widened by widen() where the value is used (a load carried
+  // across a loop then stays in flight until its use)
+  __device__ __forceinline__ uint32_t at_bits(int i, int j) const {
+    return __builtin_amdgcn_raw_buffer_load_b32(r, i * 4, j * 4, 0);
+  }
+  static __device__ __forceinline__ float widen(uint32_t b) { return __uint_as_float(b); }
   __device__ __forceinline__ void tap3(int o, int t1, int t2, float& a0, float& a1,
                                        float& a2) const {
     a0 = (*this)[o];
@@ -118,6 +124,10 @@ struct Buf<uint16_t> {
     return __uint_as_float((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, i * 2, j * 2, 0)
                            << 16);
   }
+  __device__ __forceinline__ uint32_t at_bits(int i, int j) const {
+    return __builtin_amdgcn_raw_buffer_load_b16(r, i * 2, j * 2, 0);
+  }
+  static __device__ __forceinline__ float widen(uint32_t b) { return __uint_as_float(b << 16); }
   // the 8 bytes of elements i .. i + 3 (i even, the array 4-byte aligned)
   __device__ __forceinline__ uint2 load8b(int i) const {
     return __builtin_bit_cast(uint2, __builtin_amdgcn_raw_buffer_load_b64(r, i * 2, 0, 0));

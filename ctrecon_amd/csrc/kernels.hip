@@ -854,24 +854,26 @@ __device__ __forceinline__ void bp_win_body(const BpArgs<T>& A, int col, float* 
         rec[3 * lane + 2] = make_float4(w2, w3, 0.f, 0.f);
       }
       // rows of view k0 + j: columns u0 + min(iu, nu - 1) (a repeated column has weight 0)
-      auto load_rows = [&](int j, float (&yv)[4]) {
+      // (stored bits, widened at their use: the next view's loads stay in flight while
+      // this view's voxels are summed)
+      auto load_rows = [&](int j, uint32_t (&yb)[4]) {
         const int un = __builtin_amdgcn_readlane(q.u0, j);
         const int nn = __builtin_amdgcn_readlane(q.nu, j);
         const Buf<T> pn(A.prj + (size_t)(k0 + j) * frame, frame);
 #pragma unroll
-        for (int iu = 0; iu < 4; ++iu) yv[iu] = pn.at(row, (un + min(iu, nn - 1)) * DNV);
+        for (int iu = 0; iu < 4; ++iu) yb[iu] = pn.at_bits(row, (un + min(iu, nn - 1)) * DNV);
       };
       int j = __builtin_ctzll(live);
-      float yv[4];
-      load_rows(j, yv);
+      uint32_t yb[4];
+      load_rows(j, yb);
       while (true) {
         const int nu = __builtin_amdgcn_readlane(q.nu, j);
         const int z0 = __builtin_amdgcn_readlane(q_z0, j), z1 = __builtin_amdgcn_readlane(q_z1, j);
         const float4 r0 = rec[3 * j], r1 = rec[3 * j + 1], r2 = rec[3 * j + 2];
-        float R = r1.z * yv[0];
-        R = fmaf(r1.w, yv[1], R);
-        R = fmaf(r2.x, yv[2], R);
-        R = fmaf(r2.y, yv[3], R);
+        float R = r1.z * Buf<T>::widen(yb[0]);
+        R = fmaf(r1.w, Buf<T>::widen(yb[1]), R);
+        R = fmaf(r2.x, Buf<T>::widen(yb[2]), R);
+        R = fmaf(r2.y, Buf<T>::widen(yb[3]), R);
         if (nu > 4) {
           const int un = __builtin_amdgcn_readlane(q.u0, j);
           const float fh = rdl(q.fh, j);
@@ -883,7 +885,7 @@ __device__ __forceinline__ void bp_win_body(const BpArgs<T>& A, int col, float* 
         // the next visible view's rows load while this view's voxels are summed
         live &= live - 1;
         const int jn = live ? __builtin_ctzll(live) : -1;
-        if (jn >= 0) load_rows(jn, yv);
+        if (jn >= 0) load_rows(jn, yb);
         const float rlS = r0.x, cA = r0.y, cB = r0.z, vsz = r0.w, l2f = r1.x, cws = r1.y;
         for (int zz = z0; zz <= z1; zz += 64) {
           const int z = zz + lane;
