@@ -538,6 +538,7 @@ def main():
         "traffic_source": counters.get("source"),
         "limiter": counters.get("limiter"),
         "valu_issue_frac": counters.get("valu_issue_frac"),
+        "salu_issue_frac": counters.get("salu_issue_frac"),
         "algorithmic_bytes_per_launch": per_launch_bytes,
         # compulsory I/O of one update launch: the subset's residual sinogram in, the
         # iterate and its column sums in, the iterate out
