@@ -653,6 +653,10 @@ constexpr int kStagePad = kStageRows + 2;
 constexpr int kRowsF = kStagePad + 2;  // rows, then the record table 16-byte aligned
 constexpr int kRecF = 12;
 constexpr int kStageF = kRowsF + 64 * kRecF;
+// staged waves' column sums carry 64 floats past the column: a pass's lanes past the
+// column's end (whose shadows need not be empty) add there, and the passes need no
+// per-voxel bound test
+constexpr int kLaccPad = 64;
 static_assert(kRowsF % 4 == 0 && kStageF % 4 == 0, "per-wave stage: 16-byte multiples");
 
 // bp_cell_sum over a view whose columns are already combined per row in LDS
@@ -790,7 +794,7 @@ __device__ __forceinline__ void bp_ref_body(const BpArgs<T>& A, int col, int zbl
 // adds cosVal * Wu * (vhi - vlo) (the same sum in another rounding order).
 //
 // stg (kStageF floats of this wave, 16-byte aligned, or NULL): with DNV <= kStageRows the
-// views are staged.  Lane l writes view k0 + l's record (row-shadow and cosVal constants,
+// views are staged (lacc then holds ZN + kLaccPad floats).  Lane l writes view k0 + l's record (row-shadow and cosVal constants,
 // column weights) to the wave's table; then, visible view by visible view, every lane
 // loads its row of the view's first four columns (issued while the previous view's voxels
 // are summed), combines them (R[v] = sum_u wu y[u][v], the column order and rounding of
@@ -825,6 +829,7 @@ __device__ __forceinline__ void bp_win_body(const BpArgs<T>& A, int col, float* 
   float4* rec = reinterpret_cast<float4*>(stg + kRowsF);  // [64][3] float4 records
   if (stage && lane < kStagePad - kStageRows) lR[kStageRows + lane] = 0.f;
   const int row = min(lane, DNV - 1);  // the row a lane stages
+  const float zlane = (float)lane - d.ocz;
   for (int k0 = 0; k0 < A.vc; k0 += 64) {
     const int kl = k0 + lane;
     BpView q{};
@@ -888,13 +893,16 @@ __device__ __forceinline__ void bp_win_body(const BpArgs<T>& A, int col, float* 
         if (jn >= 0) load_rows(jn, yb);
         const float rlS = r0.x, cA = r0.y, cB = r0.z, vsz = r0.w, l2f = r1.x, cws = r1.y;
         for (int zz = z0; zz <= z1; zz += 64) {
-          const int z = zz + lane;
-          const float dZ = fmaf((float)z - d.ocz, d.dz, -vsz);
+          // (float)(zz + lane) - ocz, exactly (half-integers far below 2^23); voxels past
+          // z1 lie outside the shadow's open interval with the window's 1e-2-voxel margin
+          // (empty shadows), those past the column add to lacc's padding
+          const float dZ = fmaf((float)zz + zlane, d.dz, -vsz);
           const float vlo = fmaxf(fmaf(dZ, rlS, cA), -0.5f);
           const float vhi = __builtin_elementwise_minimum(fmaf(dZ, rlS, cB), DNVh);
-          if (z <= z1 && vlo < vhi) {
+          if (vlo < vhi) {
             const float cosVal = cws * __builtin_amdgcn_sqrtf(fmaf(dZ, dZ, l2f));
-            lacc[z] = fmaf(cosVal, bp_lds_sum(lR, vlo, vhi, DNV), lacc[z]);
+            float* lz = lacc + zz;
+            lz[lane] = fmaf(cosVal, bp_lds_sum(lR, vlo, vhi, DNV), lz[lane]);
           }
         }
         if (jn < 0) break;
@@ -938,7 +946,8 @@ __global__ __launch_bounds__(kBlock) void bp_ref_kernel(BpArgs<T> A) {
 }
 
 // Windowed reference back projection; dynamic LDS: kWaves * kStageF floats of staged
-// views (DNV <= kStageRows; none otherwise), then kWaves * ZN floats of column sums.
+// views (DNV <= kStageRows; none otherwise), then kWaves * (ZN + kLaccPad) floats of
+// column sums (kWaves * ZN unstaged).
 // At most 72 VGPRs (7 waves per SIMD; no scratch).
 template <bool UPDATE, typename T>
 __global__ __launch_bounds__(kBlock, 7) void bp_win_kernel(BpArgs<T> A) {
@@ -946,7 +955,7 @@ __global__ __launch_bounds__(kBlock, 7) void bp_win_kernel(BpArgs<T> A) {
   float* lds = reinterpret_cast<float*>(lds_win);
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const bool stage = A.d.DNV <= kStageRows;
-  float* lacc = lds + (stage ? kWaves * kStageF : 0) + wid * A.d.ZN;
+  float* lacc = lds + (stage ? kWaves * kStageF : 0) + wid * (A.d.ZN + (stage ? kLaccPad : 0));
   bp_win_body<UPDATE, T>(A, blockIdx.x * kWaves + wid, lacc,
                          stage ? lds + wid * kStageF : nullptr);
 }
@@ -961,8 +970,8 @@ __global__ __launch_bounds__(kBlock) void bp_win_ones_kernel(BpArgs<float> A) {
 
 // Batched windowed reference back projection: flat work list of (scan, block of 4
 // columns); dynamic LDS: kWaves * kStageF floats of staged views (every scan of the batch
-// has DNV <= kStageRows: staged; otherwise none), then kWaves * zmax floats (zmax: the
-// tallest column of the batch).
+// has DNV <= kStageRows: staged; otherwise none), then kWaves * (zmax + kLaccPad) floats
+// (zmax: the tallest column of the batch; kWaves * zmax unstaged).
 template <typename T>
 __global__ __launch_bounds__(kBlock, 7) void bp_batch_win_kernel(const BpBatchItem<T>* __restrict__ D,
                                                               int nscan, int total, int sched,
@@ -978,7 +987,8 @@ __global__ __launch_bounds__(kBlock, 7) void bp_batch_win_kernel(const BpBatchIt
     while (s + 1 < nscan && D[s + 1].begin <= item) ++s;
     const BpBatchItem<T>& B = D[s];
     bp_win_body<true, T>(B.a, (item - B.begin) * kWaves + wid,
-                         lds + (staged ? kWaves * kStageF : 0) + wid * zmax,
+                         lds + (staged ? kWaves * kStageF : 0) +
+                             wid * (zmax + (staged ? kLaccPad : 0)),
                          staged ? lds + wid * kStageF : nullptr);
   }
 }
@@ -998,7 +1008,8 @@ __global__ __launch_bounds__(kBlock, 7) void bp_batch_win1_kernel(
   while (s + 1 < nscan && D[s + 1].begin <= i0) ++s;
   const BpBatchItem<T>& B = D[s];
   bp_win_body<true, T>(B.a, (i0 - B.begin) * kWaves + wid,
-                       lds + (staged ? kWaves * kStageF : 0) + wid * zmax,
+                       lds + (staged ? kWaves * kStageF : 0) +
+                           wid * (zmax + (staged ? kLaccPad : 0)),
                        staged ? lds + wid * kStageF : nullptr);
 }
 
@@ -1718,7 +1729,7 @@ int launch_bp(const Plan& P, const T* prj, float* vol, int vb, int vs, int vc, i
                                  accumulate ? shadow : nullptr);
   if (P.d.ZN <= kWinMaxZ) {
     const dim3 wgrid((P.d.XN * P.d.YN + kWaves - 1) / kWaves);
-    const size_t stg = P.d.DNV <= kStageRows ? kStageF : 0;  // staged views
+    const size_t stg = P.d.DNV <= kStageRows ? kStageF + kLaccPad : 0;  // staged views
     const size_t lds = (size_t)kWaves * (P.d.ZN + stg) * sizeof(float);
     if (accumulate)
       hipLaunchKernelGGL((bp_win_kernel<true, T>), wgrid, dim3(kBlock), lds, st, A);
@@ -1797,7 +1808,8 @@ int launch_bp_batch_win(const BpBatchItem<T>* items, int nscan, int total, int z
   const int sched = grid <= 0 ? 2 : (strided ? 1 : 0);
   grid = grid <= 0 ? total : std::max(1, std::min(grid, total));
   const int staged = maxDNV <= kStageRows ? 1 : 0;
-  const size_t lds = (size_t)kWaves * (zmax + (staged ? kStageF : 0)) * sizeof(float);
+  const size_t lds =
+      (size_t)kWaves * (zmax + (staged ? kStageF + kLaccPad : 0)) * sizeof(float);
   if (sched == 2)
     hipLaunchKernelGGL((bp_batch_win1_kernel<T>), dim3(grid), dim3(kBlock), lds, st, items,
                        nscan, total, zmax, staged);
