@@ -660,11 +660,12 @@ constexpr int kLaccPad = 64;
 static_assert(kRowsF % 4 == 0 && kStageF % 4 == 0, "per-wave stage: 16-byte multiples");
 
 // bp_cell_sum over a view whose columns are already combined per row in LDS
-// (lR[v] = sum_u wu y[u][v], rows past DNV zero), with the same sum: the row weights
-// min(vhi, e) - vlo, clamp(d, 0, 1), clamp(d - 1, 0, 1) of a shadow over at most three
-// rows are exactly the first / interior / last weights of edge_weights (d = vhi - e and
-// d - 1 are exact differences), and taps past the shadow add exact zeros, so the choice
-// between the two forms may be made for the whole wave.
+// (lR[v] = sum_u wu y[u][v], rows past DNV zero, two zero rows past the stage), with the
+// same sum: the row weights min(vhi, e) - vlo, clamp(d, 0, 1), clamp(d - 1, 0, 1) of a
+// shadow over at most three rows are exactly the first / interior / last weights of
+// edge_weights (d = vhi - e and d - 1 are exact differences), and taps past the shadow
+// add exact zeros (the third row's is added in every pass: most passes have a lane that
+// needs it), so the choice between the two forms may be made for the whole wave.
 __device__ __forceinline__ float bp_lds_sum(const float* lR, float vlo, float vhi, int DNV) {
 #pragma clang fp contract(off)
   const float fv = floorf(vlo + 0.5f);
@@ -677,7 +678,7 @@ __device__ __forceinline__ float bp_lds_sum(const float* lR, float vlo, float vh
     const float wv1 = __builtin_amdgcn_fmed3f(d, 0.f, 1.f);
     s = wv0 * lR[v0];
     s = fmaf(wv1, lR[v0 + 1], s);
-    if (__any(d > 1.f)) s = fmaf(__builtin_amdgcn_fmed3f(d - 1.f, 0.f, 1.f), lR[v0 + 2], s);
+    s = fmaf(__builtin_amdgcn_fmed3f(d - 1.f, 0.f, 1.f), lR[v0 + 2], s);
   } else {
     const int nv = min((int)floorf(vhi + 0.5f), DNV - 1) - v0 + 1;
     float wv0, wvl;
