@@ -346,29 +346,35 @@ __device__ __forceinline__ void fp_body(const FpArgs<TV, TO>& A, int k, int ublk
         }
       }
     } else if constexpr (COLW) {
-      // ONES: lane l evaluates the in-plane part of slice ii0 + l, read back with readlane
+      // ONES: lane l evaluates the in-plane part of slice ii0 + l, with the sum W of its
+      // cells' edge weights times mask bytes (the order of slice()), read back with
+      // readlane; a slice adds (zh - zl) W / area as slice() does
       const int lane = threadIdx.x & 63;
       for (int ii0 = 0; ii0 < NS; ii0 += 64) {
         const int il = ii0 + lane;
-        float q_r = 0.f, q_inv = 0.f, q_wj0 = 0.f, q_wjl = 0.f;
-        int q_j0 = 0, q_nj = 0;
+        float q_r = 0.f, q_inv = 0.f, q_W = 0.f;
         bool ok = false;
         SliceIP si;
         if (il < NS && fp_inplane(bl, br, ct, gload(me, il), il, NI, si)) {
           ok = true;
           q_r = si.r;
           q_inv = si.inv;
-          q_j0 = si.j0;
-          q_nj = si.nj;
-          q_wj0 = si.wj0;
-          q_wjl = si.wjl;
+          for (int ij = 0; ij < si.nj; ++ij)
+            q_W = fmaf((ij == 0) ? si.wj0 : (ij == si.nj - 1 ? si.wjl : 1.f),
+                       maskf[il * mS + (si.j0 + ij) * mC], q_W);
         }
         uint64_t live = __ballot(ok);
         while (live) {
           const int j = __builtin_ctzll(live);
           live &= live - 1;
-          slice(ii0 + j, rdl(q_r, j), rdl(q_inv, j), __builtin_amdgcn_readlane(q_j0, j),
-                __builtin_amdgcn_readlane(q_nj, j), rdl(q_wj0, j), rdl(q_wjl, j));
+          const float r = rdl(q_r, j);
+          float zl = fmaf(r, Bv, vp.azf), zh = fmaf(r, Bv1, vp.azf);
+          zl = fmaxf(zl, zlo_b);
+          zh = fminf(zh, zhi_b);
+          const bool zok = zl < zh;
+          if (!__any(zok)) continue;
+          const float W = rdl(q_W, j), inv = rdl(q_inv, j);
+          if (zok) acc = fmaf((zh - zl) * W, inv, acc);
         }
       }
     } else {
@@ -911,6 +917,36 @@ __device__ __forceinline__ void bp_win_body(const BpArgs<T>& A, int col, float* 
       }
       continue;
     }
+    if constexpr (ONES) {
+      // views' constants (with the column-weight sum Wu) from a table in LDS (stg: 64
+      // records of 8 floats), read at a uniform address instead of by readlanes
+      if (live) {
+        float4* rec1 = reinterpret_cast<float4*>(stg);
+        const int nu = q.nu;
+        const float Wu = nu == 1 ? q.wu0 : q.wu0 + (float)(nu - 2) + q.fh;
+        rec1[2 * lane] = make_float4(q.rlS, q.cA, q.cB, q.sz);
+        rec1[2 * lane + 1] = make_float4(q.l2, q.cws, Wu, 0.f);
+        while (live) {
+          const int j = __builtin_ctzll(live);
+          live &= live - 1;
+          const int z0 = __builtin_amdgcn_readlane(q_z0, j), z1 = __builtin_amdgcn_readlane(q_z1, j);
+          const float4 r0 = rec1[2 * j], r1 = rec1[2 * j + 1];
+          const float rlS = r0.x, cA = r0.y, cB = r0.z, vsz = r0.w;
+          const float l2f = r1.x, cws = r1.y, Wv = r1.z;
+          for (int zz = z0; zz <= z1; zz += 64) {
+            const int z = zz + lane;
+            const float dZ = fmaf((float)z - d.ocz, d.dz, -vsz);
+            const float vlo = fmaxf(fmaf(dZ, rlS, cA), -0.5f);
+            const float vhi = fminf(fmaf(dZ, rlS, cB), DNVh);
+            if (z <= z1 && vlo < vhi) {
+              const float cosVal = cws * __builtin_amdgcn_sqrtf(fmaf(dZ, dZ, l2f));
+              lacc[z] = fmaf(cosVal, Wv * (vhi - vlo), lacc[z]);
+            }
+          }
+        }
+      }
+      continue;
+    }
     while (live) {
       const int j = __builtin_ctzll(live);
       live &= live - 1;
@@ -961,12 +997,16 @@ __global__ __launch_bounds__(kBlock, 7) void bp_win_kernel(BpArgs<T> A) {
                          stage ? lds + wid * kStageF : nullptr);
 }
 
-// Back projection of a sinogram of ones (A.prj unused); dynamic LDS as bp_win_kernel.
-__global__ __launch_bounds__(kBlock) void bp_win_ones_kernel(BpArgs<float> A) {
-  extern __shared__ float lds_acc[];
+// Back projection of a sinogram of ones (A.prj unused); dynamic LDS: kWaves * 512
+// floats of view records, then kWaves * ZN floats of column sums.
+constexpr int kOnesRecF = 512;
+__global__ __launch_bounds__(kBlock, 8) void bp_win_ones_kernel(BpArgs<float> A) {
+  extern __shared__ float4 lds_ones[];
+  float* lds = reinterpret_cast<float*>(lds_ones);
   const int wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  bp_win_body<false, float, true>(A, blockIdx.x * kWaves + wid, lds_acc + wid * A.d.ZN,
-                                  nullptr);
+  bp_win_body<false, float, true>(A, blockIdx.x * kWaves + wid,
+                                  lds + kWaves * kOnesRecF + wid * A.d.ZN,
+                                  lds + wid * kOnesRecF);
 }
 
 // Batched windowed reference back projection: flat work list of (scan, block of 4
@@ -1676,7 +1716,7 @@ int launch_bp_ones(const Plan& P, float* vol, int vb, int vs, int vc, hipStream_
   const BpArgs<float> A = bp_args<float>(P, nullptr, vol, vb, vs, vc, nullptr, nullptr);
   const dim3 wgrid((P.d.XN * P.d.YN + kWaves - 1) / kWaves);
   hipLaunchKernelGGL(bp_win_ones_kernel, wgrid, dim3(kBlock),
-                     (size_t)kWaves * P.d.ZN * sizeof(float), st, A);
+                     (size_t)kWaves * (kOnesRecF + P.d.ZN) * sizeof(float), st, A);
   return check_launch("bp_win_ones_kernel");
 }
 
