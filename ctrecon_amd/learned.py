@@ -173,22 +173,23 @@ def _cosine(step, steps, lr):
     return lr * 0.5 * (1.0 + math.cos(math.pi * min(step, steps) / steps))
 
 
+def running_std(arrays) -> float:
+    """Standard deviation over all elements of several arrays, accumulated array by array
+    in FP64 (no concatenated copy of a clinical-size training set)."""
+    n, s1, s2 = 0, 0.0, 0.0
+    for a in arrays:
+        a64 = a.astype(np.float64, copy=False).ravel()
+        n += a64.size
+        s1 += float(a64.sum())
+        s2 += float(np.dot(a64, a64))
+    mean = s1 / n
+    return float(np.sqrt(max(s2 / n - mean * mean, 0.0)))
+
+
 def train(pairs, steps=2000, crop=(128, 128, 64), lr=1e-3, seed=0, val=None, log=_log):
     """Adam on random crops of the training pairs; returns the trained UNet."""
-    # standard deviations over all pairs, accumulated pair by pair (no concatenated copy
-    # of the clinical-size training set)
-    def _std(arrays):
-        n, s1, s2 = 0, 0.0, 0.0
-        for a in arrays:
-            a64 = a.astype(np.float64, copy=False).ravel()
-            n += a64.size
-            s1 += float(a64.sum())
-            s2 += float(np.dot(a64, a64))
-        mean = s1 / n
-        return float(np.sqrt(max(s2 / n - mean * mean, 0.0)))
-
-    s_d = 1.0 / max(_std(p[1] for p in pairs), 1e-12)
-    s_out = max(_std(p[2] for p in pairs), 1e-12)
+    s_d = 1.0 / max(running_std(p[1] for p in pairs), 1e-12)
+    s_out = max(running_std(p[2] for p in pairs), 1e-12)
     net = UNet(init_params(seed), (1.0, s_d, s_out))
     rng = np.random.default_rng(seed)
     dx, dd, dt = (DeviceArray(crop) for _ in range(3))
