@@ -10,11 +10,6 @@ namespace ctr {
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 
-// Overlap of [lo, hi] with the cell [c - 0.5, c + 0.5].
-__device__ __forceinline__ float cell_w(float lo, float hi, float c) {
-  return fminf(hi, c + 0.5f) - fmaxf(lo, c - 0.5f);
-}
-
 // BF16 storage: the upper 16 bits of an FP32 value, held as uint16_t.  Values are
 // widened to FP32 on load and rounded to nearest even on store (v_cvt_pk_bf16_f32);
 // all arithmetic is FP32 or wider.
